@@ -1,0 +1,271 @@
+"""Python binding of libnqm.so (the C ABI declared in include/nqm.h).
+
+A thin ctypes layer for tests, the bench and scripting: every call goes
+through the C entry points, which run the HIP kernels. There is no CPU
+fallback. If libnqm.so is missing or no wave64 GPU is present, the calls
+raise.
+
+Usage:
+    import sys; sys.path.insert(0, "<repo>/gpu-n-queens_amd"); import nqm
+    with nqm.Context(0) as ctx:
+        total = ctx.count_pool(16, 5, pool)          # pool: uint8 (count, 5)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnqm.so")
+
+OK = 0
+EINVAL = 1
+EPREFIX = 2
+ENOMEM = 3
+EDEVICE = 4
+ERANGE = 5
+EIO = 6
+EFORMAT = 7
+EFLAGS = 8
+EUNSUPPORTED = 9
+
+MIRROR_WEIGHT = 0x1
+
+U8P = ctypes.POINTER(ctypes.c_uint8)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+VP = ctypes.c_void_p
+
+
+class NqmError(RuntimeError):
+    def __init__(self, status, detail=""):
+        self.status = status
+        msg = f"nqm status {status}"
+        if _lib is not None:
+            msg += f" ({_lib.nqm_strerror(status).decode()})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
+                ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("launches", ctypes.c_uint32), ("blocks", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GpuConfig(ctypes.Structure):
+    """gpu_config_t of the reference (GPU_NQueens.cuh:8-11)."""
+    _fields_ = [("device_id", ctypes.c_uint), ("async_", ctypes.c_int)]
+
+
+_lib = None
+
+# Exported symbols and their (argtypes, restype); the not-gpu tests check that
+# the library exports each of them.
+_SIGNATURES = {
+    "nqm_create": ([ctypes.c_int, ctypes.POINTER(VP)], ctypes.c_int),
+    "nqm_destroy": ([VP], None),
+    "nqm_strerror": ([ctypes.c_int], ctypes.c_char_p),
+    "nqm_last_error": ([VP], ctypes.c_char_p),
+    "nqm_device_info": ([VP] + [ctypes.POINTER(ctypes.c_uint32)] * 5, ctypes.c_int),
+    "nqm_count_pool": ([VP, ctypes.c_uint32, ctypes.c_uint32, U8P, ctypes.c_uint64,
+                        ctypes.c_uint32, U64P], ctypes.c_int),
+    "nqm_count_pool_device": ([VP, ctypes.c_uint32, ctypes.c_uint32, VP, ctypes.c_uint64,
+                               ctypes.c_uint32, VP, VP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_solver_driver": ([ctypes.c_uint32, VP, ctypes.c_uint32, ctypes.c_uint,
+                           ctypes.POINTER(GpuConfig), ctypes.c_uint, U64P], ctypes.c_int),
+    "nqm_state_size": ([ctypes.c_uint32], ctypes.c_size_t),
+    "nqm_dev_alloc": ([VP, ctypes.c_size_t, ctypes.POINTER(VP)], ctypes.c_int),
+    "nqm_dev_free": ([VP, VP], ctypes.c_int),
+    "nqm_memcpy_h2d": ([VP, VP, VP, ctypes.c_size_t], ctypes.c_int),
+    "nqm_memcpy_d2h": ([VP, VP, VP, ctypes.c_size_t], ctypes.c_int),
+}
+
+
+def load():
+    """Load libnqm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run gpu-n-queens_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def strerror(status):
+    return load().nqm_strerror(status).decode()
+
+
+def state_size(n):
+    return int(load().nqm_state_size(n))
+
+
+def _pool_array(pool, depth):
+    a = np.ascontiguousarray(pool, dtype=np.uint8)
+    if depth:
+        a = a.reshape(-1, depth)
+    return a
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (nqm_dev_alloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = VP()
+        ctx._check(ctx.lib.nqm_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx, array):
+        a = np.ascontiguousarray(array)
+        buf = cls(ctx, max(a.nbytes, 1))
+        if a.nbytes:
+            ctx._check(ctx.lib.nqm_memcpy_h2d(ctx.handle, buf.ptr, a.ctypes.data, a.nbytes))
+        return buf
+
+    def to_host(self, dtype, count):
+        out = np.zeros(count, dtype=dtype)
+        if out.nbytes:
+            self.ctx._check(self.ctx.lib.nqm_memcpy_d2h(self.ctx.handle, out.ctypes.data,
+                                                        self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.nqm_dev_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """One nqm_ctx (device, stream, events, cached device buffers)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = VP()
+        rc = self.lib.nqm_create(device, ctypes.byref(h))
+        if rc != OK:
+            raise NqmError(rc, f"nqm_create(device={device})")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self.lib.nqm_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise NqmError(rc, self.lib.nqm_last_error(self.handle).decode())
+        return rc
+
+    def last_error(self):
+        return self.lib.nqm_last_error(self.handle).decode()
+
+    def device_info(self):
+        vals = [ctypes.c_uint32() for _ in range(5)]
+        self._check(self.lib.nqm_device_info(self.handle, *[ctypes.byref(v) for v in vals]))
+        keys = ["cus", "wave_size", "clock_khz", "lds_per_block", "lds_per_cu"]
+        return {k: v.value for k, v in zip(keys, vals)}
+
+    # -- the counting path -------------------------------------------------
+    def count_pool(self, n, depth, pool, flags=0, count=None):
+        """Total (weighted) solutions below a host pool (nqm_count_pool)."""
+        a = _pool_array(pool, depth)
+        cnt = a.shape[0] if count is None else count
+        tot = ctypes.c_uint64()
+        src = a if a.size else np.zeros(1, np.uint8)
+        self._check(self.lib.nqm_count_pool(self.handle, n, depth, src.ctypes.data_as(U8P), cnt,
+                                            flags, ctypes.byref(tot)))
+        return tot.value
+
+    def count_pool_raw(self, n, depth, pool, flags=0, count=None):
+        """Like count_pool but returns the raw status instead of raising."""
+        a = _pool_array(pool, depth)
+        cnt = a.shape[0] if count is None else count
+        tot = ctypes.c_uint64()
+        src = a if a.size else np.zeros(1, np.uint8)
+        rc = self.lib.nqm_count_pool(self.handle, n, depth, src.ctypes.data_as(U8P), cnt, flags,
+                                     ctypes.byref(tot))
+        return rc, tot.value
+
+    def count_pool_device(self, n, depth, d_pool, count, flags=0, per_prefix=False):
+        """Count a device-resident pool. Returns Stats, plus per-prefix
+        (unweighted) solution and node arrays when per_prefix is set."""
+        st = Stats()
+        ps = pn = None
+        if per_prefix:
+            ps = DeviceBuffer(self, 8 * count)
+            pn = DeviceBuffer(self, 8 * count)
+        try:
+            self._check(self.lib.nqm_count_pool_device(
+                self.handle, n, depth, d_pool.ptr if isinstance(d_pool, DeviceBuffer) else d_pool,
+                count, flags, ps.ptr if ps else None, pn.ptr if pn else None, ctypes.byref(st)))
+            if per_prefix:
+                return st, ps.to_host(np.uint64, count), pn.to_host(np.uint64, count)
+            return st
+        finally:
+            if ps:
+                ps.free()
+            if pn:
+                pn.free()
+
+
+def states_from_pool(n, lock, pool):
+    """Build an array in the reference's nq_state_t layout (n_queens.cuh:100-113)
+    from a compact pool, as the reference's loader does (nq_utils.cu:244-251)."""
+    pool = _pool_array(pool, lock)
+    sz = state_size(n)
+    out = np.zeros((pool.shape[0], sz), np.uint8)
+    for i, pre in enumerate(pool):
+        diag = anti = 0
+        cols = 0
+        for r, c in enumerate(pre):
+            c = int(c)
+            cols |= 1 << c
+            diag |= (1 << c) << r
+            anti |= (1 << c) << (n - r)
+        out[i, 0:8] = np.frombuffer(np.uint64(diag).tobytes(), np.uint8)
+        out[i, 8:16] = np.frombuffer(np.uint64(anti).tobytes(), np.uint8)
+        out[i, 16:20] = np.frombuffer(np.uint32(cols).tobytes(), np.uint8)
+        out[i, 20] = lock
+        out[i, 21:21 + n] = 0xFF
+        out[i, 21:21 + lock] = pre
+    return out
+
+
+def solver_driver(n, states, locked_row_end, device=0):
+    """nqm_solver_driver: the reference's gpu_solver_driver argument layout."""
+    L = load()
+    states = np.ascontiguousarray(states, dtype=np.uint8)
+    cfg = GpuConfig(device, 0)
+    tot = ctypes.c_uint64()
+    rc = L.nqm_solver_driver(n, states.ctypes.data, states.shape[0], locked_row_end,
+                             ctypes.byref(cfg), 1, ctypes.byref(tot))
+    if rc != OK:
+        raise NqmError(rc, "nqm_solver_driver")
+    return tot.value
