@@ -1,0 +1,447 @@
+// k2_queue.hip -- K2: persistent work-queue counting for gfx950 (wave64).
+//
+// The reference statically gives each thread one pool state and lets a warp
+// idle until its slowest lane finishes (GPU_NQueens.cu:338-360; the restart
+// idea is future work in README.md:46-56). K2 keeps the same arithmetic (the
+// per-lane placement step of K1, same counts, same node numbers) and changes
+// only the scheduling:
+//
+//  * k2_expand: items at row r -> their children at row r+1, until each
+//    solve's items reach its target row. One thread per item; children are
+//    reserved with one atomic per wave (wave prefix sum). Dead children
+//    (next row without a free cell) are dropped; every placement is counted
+//    as a node of its solve. Item counts live in device memory, so a chain
+//    of expand launches needs no host round trip (and can be graph-captured).
+//
+//  * k2_drain: a persistent grid. Lanes draw items from a global queue
+//    (one atomicAdd per wave refill for all idle lanes, ranks by mbcnt).
+//    When the queue is empty, idle lanes take work from busy lanes of the
+//    same wave: a donor hands over the bottom entry of its LDS stack (the
+//    shallowest open row, i.e. the largest untried subtree) and simply moves
+//    its stack base up; the receiver reads that 16-byte entry straight from
+//    the donor's LDS slot. Solves of different n mix freely: each lane holds
+//    its own column mask. Counts are folded per lane and flushed to per-solve
+//    u64 counters when the lane switches solve or exits.
+//
+//  * Optional pass budget: when a wave has run `budget` passes, each live
+//    lane writes its open rows (current row + stack entries) to a spill
+//    queue as independent items, and the wave exits. The host relaunches on
+//    the spill queue (restart), which also lets new work join between rounds.
+
+#include "nqm_internal.h"
+#include "nqm_items.h"
+
+namespace nqm {
+
+constexpr uint32_t kK2Threads = 256;
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// expand: one level of children for every item below its solve's target row
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
+                 Item* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
+                 const uint8_t* __restrict__ target_row, uint64_t* __restrict__ solve_nodes,
+                 uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err) {
+    const uint32_t cnt = *in_count;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t step = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += step) {
+        const uint32_t i = base + threadIdx.x;
+        Item it{};
+        uint32_t nch = 0, n = 0, row = 0, mask = 0, placed = 0;
+        bool expand = false;
+        if (i < cnt) {
+            it = in[i];
+            n = meta_n(it.meta);
+            row = meta_row(it.meta);
+            mask = (1u << n) - 1u;
+            expand = row < target_row[it.solve];
+            if (expand) {
+                uint32_t f = it.f;
+                placed = __popc(f);
+                while (f) {
+                    const uint32_t b = f & (0u - f);
+                    f ^= b;
+                    const uint32_t cf = ~((it.cols | b) | ((it.ld | b) << 1) | ((it.rd | b) >> 1)) & mask;
+                    nch += (cf != 0u);
+                }
+            } else {
+                nch = 1;
+            }
+        }
+        // nodes placed by this level, weighted to full-tree numbers
+        if (expand) {
+            atomicAdd((unsigned long long*)&solve_nodes[it.solve],
+                      (unsigned long long)placed << meta_wshift(it.meta));
+        }
+        const uint64_t wplaced = wave_sum64(placed);
+        if (lane == 0 && wplaced) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wplaced);
+        const uint32_t incl = wave_incl_scan(nch, lane);
+        const uint32_t total = __shfl(incl, 63, 64);
+        uint32_t obase = 0;
+        if (lane == 0 && total) obase = atomicAdd(out_count, total);
+        obase = __shfl(obase, 0, 64);
+        uint32_t o = obase + incl - nch;
+        if (nch && (uint64_t)obase + total > out_cap) {
+            atomicOr(err, 2u);
+            continue;
+        }
+        if (i < cnt) {
+            if (!expand) {
+                out[o] = it;
+            } else {
+                uint32_t f = it.f;
+                const uint32_t ws = meta_wshift(it.meta);
+                while (f) {
+                    const uint32_t b = f & (0u - f);
+                    f ^= b;
+                    Item c;
+                    c.cols = it.cols | b;
+                    c.ld = ((it.ld | b) << 1) & mask;
+                    c.rd = (it.rd | b) >> 1;
+                    c.f = ~(c.cols | c.ld | c.rd) & mask;
+                    c.solve = it.solve;
+                    c.meta = item_meta(n, row + 1, ws);
+                    c.pad0 = c.pad1 = 0;
+                    if (c.f) out[o++] = c;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// drain: persistent queue + intra-wave donation (+ optional spill budget)
+// ---------------------------------------------------------------------------
+struct DrainArgs {
+    const Item* items;
+    const uint32_t* item_count;
+    uint32_t* head;
+    Item* spill;
+    uint32_t* spill_count;
+    uint32_t spill_cap;
+    uint64_t* solve_sols;
+    uint64_t* solve_nodes;
+    uint64_t* exec_nodes;
+    uint32_t* err;
+    uint32_t stack_rows;
+    uint32_t refill_idle;
+    uint32_t donate;
+    uint32_t pad;
+    uint64_t budget;  // 0: drain everything
+};
+
+__device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, uint64_t& sols,
+                                          uint64_t& nodes) {
+    if (sols) atomicAdd((unsigned long long*)&a.solve_sols[solve], (unsigned long long)sols);
+    if (nodes) atomicAdd((unsigned long long*)&a.solve_nodes[solve], (unsigned long long)nodes);
+    sols = 0;
+    nodes = 0;
+}
+
+__global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
+    extern __shared__ uint4 lds[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t stride = blockDim.x;
+    uint4* const stack = lds;                                   // [stack_rows][stride]
+    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds + (size_t)a.stack_rows * stride) +
+                               wave * 64u;                      // 64 u32 per wave
+    const uint32_t count = *a.item_count;
+
+    // lane state
+    uint32_t f = 0, cols = 0, ld = 0, rd = 0, mask = 0, solve = 0, ws = 0;
+    uint32_t sbase = 0, stop = 0;  // stack slots [sbase, stop)
+    bool live = false;
+    uint64_t start = 0, isols = 0;
+    uint32_t acc_solve = 0;
+    uint64_t acc_sols = 0, acc_nodes = 0, exec = 0;
+
+    bool queue_open = true;
+    uint64_t pass = 0, next_refill = 0;
+
+    for (;;) {
+        // (1) exhausted row: count a full board, then pop or finish the piece
+        if (live && f == 0) {
+            isols += (cols == mask);
+            if (stop == sbase) {
+                live = false;
+                const uint64_t nd = pass - start;
+                exec += nd;
+                acc_nodes += nd << ws;
+                acc_sols += isols << ws;
+                isols = 0;
+            } else {
+                --stop;
+                const uint4 e = stack[stop * stride + tid];
+                f = e.x;
+                cols = e.y;
+                ld = e.z;
+                rd = e.w;
+            }
+        }
+
+        // (2) refill idle lanes: global queue first, then donation in-wave
+        uint64_t idle = __ballot(!live);
+        const uint32_t nidle = __popcll(idle);
+        if (nidle && (nidle == 64u || (nidle >= a.refill_idle && pass >= next_refill))) {
+            if (queue_open) {
+                uint32_t qbase = 0;
+                if (lane == 0) qbase = atomicAdd(a.head, nidle);
+                qbase = __shfl(qbase, 0, 64);
+                const uint32_t avail = qbase >= count ? 0u : min(nidle, count - qbase);
+                if (avail < nidle) queue_open = false;
+                const uint32_t r = lane_rank(idle);
+                if (!live && r < avail) {
+                    const Item it = a.items[qbase + r];
+                    if (it.solve != acc_solve) {
+                        flush_acc(a, acc_solve, acc_sols, acc_nodes);
+                        acc_solve = it.solve;
+                    }
+                    f = it.f;
+                    cols = it.cols;
+                    ld = it.ld;
+                    rd = it.rd;
+                    solve = it.solve;
+                    mask = (1u << meta_n(it.meta)) - 1u;
+                    ws = meta_wshift(it.meta);
+                    sbase = stop = 0;
+                    start = pass;
+                    isols = 0;
+                    live = true;
+                }
+                idle = __ballot(!live);
+            }
+            if (idle && a.donate) {
+                const uint64_t donors = __ballot(live && stop > sbase);
+                const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
+                if (k) {
+                    const bool is_donor = live && stop > sbase;
+                    const uint32_t dr = lane_rank(donors);
+                    if (is_donor && dr < k) wscratch[dr] = lane;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const uint32_t ir = lane_rank(idle);
+                    const bool recv = !live && ir < k;
+                    const uint32_t dl = recv ? wscratch[ir] : lane;
+                    const uint32_t d_base = __shfl(sbase, dl, 64);
+                    const uint32_t d_mask = __shfl(mask, dl, 64);
+                    const uint32_t d_solve = __shfl(solve, dl, 64);
+                    const uint32_t d_ws = __shfl(ws, dl, 64);
+                    if (recv) {
+                        const uint4 e = stack[d_base * stride + (wave * 64u + dl)];
+                        if (d_solve != acc_solve) {
+                            flush_acc(a, acc_solve, acc_sols, acc_nodes);
+                            acc_solve = d_solve;
+                        }
+                        f = e.x;
+                        cols = e.y;
+                        ld = e.z;
+                        rd = e.w;
+                        mask = d_mask;
+                        solve = d_solve;
+                        ws = d_ws;
+                        sbase = stop = 0;
+                        start = pass;
+                        isols = 0;
+                        live = true;
+                    }
+                    if (is_donor && dr < k) ++sbase;
+                    // keep the scratch stable until every receiver has read it
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            const uint32_t still = __popcll(__ballot(!live));
+            next_refill = (still >= a.refill_idle && still < 64u) ? pass + 64u : pass;
+        }
+
+        if (__ballot(live) == 0) {
+            if (!queue_open) break;
+            continue;  // an all-idle wave refills on the next pass
+        }
+
+        // (3) pass budget: spill every open row as an item, then leave
+        if (a.budget && pass >= a.budget) {
+            if (live) {
+                const uint32_t nitems = 1u + (stop - sbase);
+                const uint32_t o = atomicAdd(a.spill_count, nitems);
+                if ((uint64_t)o + nitems > a.spill_cap) {
+                    atomicOr(a.err, 4u);
+                } else {
+                    const uint32_t meta_base = item_meta(31u - __clz(mask + 1u), 0u, ws);
+                    Item it;
+                    it.solve = solve;
+                    it.pad0 = it.pad1 = 0;
+                    it.f = f;
+                    it.cols = cols;
+                    it.ld = ld;
+                    it.rd = rd;
+                    it.meta = meta_base | ((uint32_t)__popc(cols) << 8);
+                    a.spill[o] = it;
+                    for (uint32_t s = sbase; s < stop; ++s) {
+                        const uint4 e = stack[s * stride + tid];
+                        it.f = e.x;
+                        it.cols = e.y;
+                        it.ld = e.z;
+                        it.rd = e.w;
+                        it.meta = meta_base | ((uint32_t)__popc(e.y) << 8);
+                        a.spill[o + 1u + (s - sbase)] = it;
+                    }
+                }
+                const uint64_t nd = pass - start;
+                exec += nd;
+                acc_nodes += nd << ws;
+                acc_sols += isols << ws;
+                isols = 0;
+                live = false;
+            }
+            break;
+        }
+
+        // (4) one placement per live lane (the K1 step)
+        if (live) {
+            const uint32_t b = f & (0u - f);
+            f ^= b;
+            if (f) {
+                stack[stop * stride + tid] = make_uint4(f, cols, ld, rd);
+                ++stop;
+            }
+            cols |= b;
+            ld = (ld | b) << 1;
+            rd = (rd | b) >> 1;
+            f = ~(cols | ld | rd) & mask;
+        }
+        ++pass;
+    }
+
+    flush_acc(a, acc_solve, acc_sols, acc_nodes);
+    exec = wave_sum64(exec);
+    if (lane == 0 && exec) atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec);
+}
+
+// ---------------------------------------------------------------------------
+// small helpers (graph-safe clears instead of memset nodes)
+// ---------------------------------------------------------------------------
+__global__ void k2_fill_u32_kernel(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k2_fill_u64_kernel(uint64_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
+    const uint32_t blocks = n ? min(1024u, (n + 255u) / 256u) : 1u;
+    hipLaunchKernelGGL(k2_fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n) {
+    const uint32_t blocks = n ? min(1024u, (n + 255u) / 256u) : 1u;
+    hipLaunchKernelGGL(k2_fill_u64_kernel, dim3(blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
+                     Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
+                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err) {
+    hipLaunchKernelGGL(k2_expand_kernel, dim3(grid), dim3(256), 0, s, in, in_count, out, out_count,
+                       out_cap, target_row, solve_nodes, exec_nodes, err);
+    return hipGetLastError();
+}
+
+// Items a budget-ended round never handed out: items[head .. count) move to
+// the end of the spill queue so the next round sees them.
+__global__ void k2_append_leftover_kernel(const Item* __restrict__ items,
+                                          const uint32_t* __restrict__ count,
+                                          const uint32_t* __restrict__ head, Item* __restrict__ out,
+                                          uint32_t* __restrict__ out_count, uint32_t out_cap,
+                                          uint32_t* __restrict__ err) {
+    const uint32_t c = *count;
+    const uint32_t h = min(*head, c);
+    const uint32_t left = c - h;
+    const uint32_t o = *out_count;  // stable: the drain round has ended
+    if (left == 0) return;
+    if ((uint64_t)o + left > out_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 8u);
+        return;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < left; i += gridDim.x * blockDim.x)
+        out[o + i] = items[h + i];
+}
+
+__global__ void k2_bump_count_kernel(const uint32_t* __restrict__ count,
+                                     const uint32_t* __restrict__ head, uint32_t* __restrict__ out_count,
+                                     uint32_t out_cap) {
+    const uint32_t c = *count;
+    const uint32_t h = min(*head, c);
+    if ((uint64_t)*out_count + (c - h) <= out_cap) *out_count += c - h;
+}
+
+hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,
+                              const uint32_t* head, Item* out, uint32_t* out_count, uint32_t out_cap,
+                              uint32_t* err) {
+    hipLaunchKernelGGL(k2_append_leftover_kernel, dim3(1024), dim3(256), 0, s, items, count, head, out,
+                       out_count, out_cap, err);
+    hipLaunchKernelGGL(k2_bump_count_kernel, dim3(1), dim3(1), 0, s, count, head, out_count, out_cap);
+    return hipGetLastError();
+}
+
+size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads) {
+    return (size_t)stack_rows * threads * sizeof(uint4) + (size_t)threads * sizeof(uint32_t);
+}
+
+hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p) {
+    DrainArgs a;
+    a.items = p.items;
+    a.item_count = p.item_count;
+    a.head = p.head;
+    a.spill = p.spill;
+    a.spill_count = p.spill_count;
+    a.spill_cap = p.spill_cap;
+    a.solve_sols = p.solve_sols;
+    a.solve_nodes = p.solve_nodes;
+    a.exec_nodes = p.exec_nodes;
+    a.err = p.err;
+    a.stack_rows = p.stack_rows;
+    a.refill_idle = p.refill_idle;
+    a.donate = p.donate;
+    a.pad = 0;
+    a.budget = p.budget;
+    hipLaunchKernelGGL(k2_drain_kernel, dim3(blocks), dim3(threads),
+                       k2_lds_bytes(p.stack_rows, threads), s, a);
+    return hipGetLastError();
+}
+
+int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel, (int)threads, lds_bytes) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return NQM_EDEVICE;
+    }
+    *out = nb;
+    return NQM_OK;
+}
+
+}  // namespace nqm

@@ -70,7 +70,7 @@ static void free_host(HostBuf& b) {
     b.cap = 0;
 }
 
-static double now_ms() {
+double now_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }

@@ -1,0 +1,383 @@
+// nqm_batch.hip -- nqm_solve_batch: many solves in one persistent pass (K2).
+//
+// Host plan:
+//   1. each solve's prefix becomes one item at row k (an empty board of
+//      n >= 2 becomes ceil(n/2) row-1 items, the left half weighted x2, as
+//      the reference's mirror halving, deffinitions.cuh:26-30);
+//   2. k2_expand grows items level by level on the device until each solve
+//      reaches its target row (explicit depth) or the queue holds enough
+//      items to fill the chip (automatic depth);
+//   3. k2_drain empties the queue; with a pass budget, live work is spilled
+//      and the drain relaunched until nothing is left (restart);
+//   4. per-solve u64 counters come back in one copy.
+// With NQM_TUNE_GRAPH, steps 2-4 are captured once into a HIP graph (the
+// level sizes learned by a first, uncaptured expansion) and launched as one
+// graph; counters are cleared by fill kernels, never memset nodes.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "nqm_internal.h"
+#include "nqm_items.h"
+
+namespace nqm {
+
+namespace {
+
+struct HostInit {
+    std::vector<Item> items;
+    std::vector<uint64_t> sols;   // solutions found while building items (full boards)
+    std::vector<uint64_t> nodes;  // weighted placements made on the host
+    std::vector<uint8_t> row;     // row of each solve's initial items
+    uint64_t exec_nodes = 0;      // unweighted placements made on the host
+};
+
+// Relative bitsets of a prefix at row k; false if the prefix is invalid.
+bool prefix_state(uint32_t n, uint32_t k, const uint8_t* cols_in, uint32_t* cols, uint32_t* ld,
+                  uint32_t* rd) {
+    const uint32_t mask = (1u << n) - 1u;
+    uint32_t c = 0, l = 0, r = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        if (cols_in[i] >= n) return false;
+        const uint32_t b = 1u << cols_in[i];
+        if (b & (c | l | r)) return false;
+        c |= b;
+        l = ((l | b) << 1) & mask;
+        r = (r | b) >> 1;
+    }
+    *cols = c;
+    *ld = l;
+    *rd = r;
+    return true;
+}
+
+int build_host_items(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, bool mirror, HostInit* h) {
+    h->sols.assign(ns, 0);
+    h->nodes.assign(ns, 0);
+    h->row.assign(ns, 0);
+    for (uint32_t s = 0; s < ns; ++s) {
+        const nqm_solve& q = solves[s];
+        if (q.n < 1 || q.n > kMaxN || q.k > q.n || q.k > 32)
+            return fail(ctx, NQM_EINVAL, "solve %u: n=%u k=%u out of range", s, q.n, q.k);
+        const uint32_t mask = (1u << q.n) - 1u;
+        uint32_t cols, ld, rd;
+        if (!prefix_state(q.n, q.k, q.prefix, &cols, &ld, &rd))
+            return fail(ctx, NQM_EPREFIX, "solve %u: prefix is not a valid placement", s);
+        h->row[s] = (uint8_t)q.k;
+        if (q.k == q.n) {  // a complete board
+            h->sols[s] = 1;
+            continue;
+        }
+        if (q.k == 0 && mirror && q.n >= 2) {
+            // row-0 columns < ceil(n/2); x2 weight left of n/2 (integer div)
+            h->row[s] = 1;
+            for (uint32_t c = 0; c < (q.n + 1) / 2; ++c) {
+                const uint32_t ws = c < q.n / 2 ? 1u : 0u;
+                const uint32_t b = 1u << c;
+                Item it{};
+                it.cols = b;
+                it.ld = (b << 1) & mask;
+                it.rd = b >> 1;
+                it.f = ~(it.cols | it.ld | it.rd) & mask;
+                it.solve = s;
+                it.meta = item_meta(q.n, 1, ws);
+                h->nodes[s] += 1ull << ws;
+                h->exec_nodes += 1;
+                if (it.f) h->items.push_back(it);
+            }
+            continue;
+        }
+        Item it{};
+        it.cols = cols;
+        it.ld = ld;
+        it.rd = rd;
+        it.f = ~(cols | ld | rd) & mask;
+        it.solve = s;
+        it.meta = item_meta(q.n, q.k, 0);
+        if (it.f) h->items.push_back(it);
+    }
+    return NQM_OK;
+}
+
+// Device buffers of one batch call; freed on every exit path.
+struct Batch {
+    DevBuf init, q[2], counts, head, spill_count, err, target, sols, nodes, exec;
+    ~Batch() {
+        for (DevBuf* d : {&init, &q[0], &q[1], &counts, &head, &spill_count, &err, &target, &sols,
+                          &nodes, &exec})
+            if (d->ptr) (void)hipFree(d->ptr);
+    }
+};
+
+// K2 drain grid for a stack depth: as many waves per CU as LDS and the
+// requested cap allow.
+int drain_shape(nqm_ctx* ctx, uint32_t stack_rows, const nqm_tune& t, uint32_t* blocks,
+                uint32_t* threads) {
+    uint32_t th = t.threads ? t.threads : 256u;
+    if (th != 64 && th != 128 && th != 256) return fail(ctx, NQM_EINVAL, "threads=%u", th);
+    while (th > 64 && k2_lds_bytes(stack_rows, th) > ctx->lds_per_block) th >>= 1;
+    if (k2_lds_bytes(stack_rows, th) > ctx->lds_per_block)
+        return fail(ctx, NQM_EUNSUPPORTED, "stack of %u rows does not fit LDS", stack_rows);
+    int per_cu = 0;
+    int rc = k2_max_blocks_per_cu(th, k2_lds_bytes(stack_rows, th), &per_cu);
+    if (rc != NQM_OK) return fail(ctx, rc, "occupancy query failed");
+    if (per_cu < 1) per_cu = 1;
+    if (t.waves_per_cu) {
+        const int cap = (int)std::max(1u, t.waves_per_cu / (th / 64u));
+        per_cu = std::min(per_cu, cap);
+    }
+    *blocks = (uint32_t)per_cu * (uint32_t)ctx->prop.multiProcessorCount;
+    *threads = th;
+    return NQM_OK;
+}
+
+}  // namespace
+
+int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const nqm_tune* tune_in,
+                     uint64_t* counts, uint64_t* nodes_out, nqm_stats* stats) {
+    const double t0 = now_ms();
+    nqm_tune t{};
+    if (tune_in) t = *tune_in;
+    if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH))
+        return fail(ctx, NQM_EINVAL, "unknown tune flags 0x%x", t.flags);
+    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 16u;
+    const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
+    const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
+
+    HostInit h;
+    int rc = build_host_items(ctx, solves, ns, mirror, &h);
+    if (rc != NQM_OK) return rc;
+
+    uint32_t max_n = 1;
+    for (uint32_t s = 0; s < ns; ++s) max_n = std::max(max_n, solves[s].n);
+
+    // Target rows. Explicit depth: min(n-1, max(row, depth)). Automatic:
+    // expand toward n-4 but stop once the queue can fill the chip. A
+    // stack-row cap raises the target so that n-1-row <= stack_rows.
+    std::vector<uint8_t> target(ns);
+    const bool auto_depth = t.depth == 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        const uint32_t n = solves[s].n, r = h.row[s];
+        uint32_t tr = auto_depth ? (n > 4 ? std::max(r, n - 4) : r)
+                                 : std::max(r, std::min(t.depth, n - 1));
+        if (t.stack_rows && n - 1 > t.stack_rows + tr) tr = n - 1 - t.stack_rows;
+        target[s] = (uint8_t)std::min(tr, n - 1);
+    }
+    const uint64_t want_items =
+        auto_depth ? (uint64_t)ctx->prop.multiProcessorCount * 16u * 64u * 8u : UINT64_MAX;
+
+    Batch b;
+    const uint32_t count0 = (uint32_t)h.items.size();
+    if ((rc = ensure_dev(ctx, b.init, std::max<size_t>(count0, 1) * sizeof(Item))) != NQM_OK) return rc;
+    for (DevBuf* d : {&b.counts, &b.head, &b.spill_count, &b.err, &b.exec})
+        if ((rc = ensure_dev(ctx, *d, 64)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.target, std::max<size_t>(ns, 1))) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.sols, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.nodes, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
+
+    uint32_t* const d_cnt = static_cast<uint32_t*>(b.counts.ptr);  // 16 slots
+    uint32_t* const d_head = static_cast<uint32_t*>(b.head.ptr);
+    uint32_t* const d_spill_count = static_cast<uint32_t*>(b.spill_count.ptr);
+    uint32_t* const d_err = static_cast<uint32_t*>(b.err.ptr);
+    uint64_t* const d_exec = static_cast<uint64_t*>(b.exec.ptr);
+    uint64_t* const d_sols = static_cast<uint64_t*>(b.sols.ptr);
+    uint64_t* const d_nodes = static_cast<uint64_t*>(b.nodes.ptr);
+    const uint8_t* const d_target = static_cast<uint8_t*>(b.target.ptr);
+    uint32_t* const d_qcount = d_cnt + 12;  // the drain reads its queue size here
+    hipStream_t st = ctx->stream;
+
+    NQM_HIP(ctx, hipMemcpyAsync(b.target.ptr, target.data(), ns, hipMemcpyHostToDevice, st));
+    if (count0)
+        NQM_HIP(ctx, hipMemcpyAsync(b.init.ptr, h.items.data(), count0 * sizeof(Item),
+                                    hipMemcpyHostToDevice, st));
+
+    // ---- plan: expand level by level, learning each level's size ---------
+    // Level L reads slot L (slot 0 = initial count) and writes slot L+1;
+    // buffers: init -> q[0] -> q[1] -> q[0] ...
+    auto level_in = [&](uint32_t L) -> Item* {
+        return L == 0 ? static_cast<Item*>(b.init.ptr) : static_cast<Item*>(b.q[(L - 1) & 1].ptr);
+    };
+    auto level_out = [&](uint32_t L) -> Item* { return static_cast<Item*>(b.q[L & 1].ptr); };
+    auto enqueue_clear = [&]() -> hipError_t {
+        hipError_t e;
+        if ((e = k2_fill_u32(st, d_cnt, 16, 0)) != hipSuccess) return e;
+        if ((e = k2_fill_u32(st, d_cnt, 1, count0)) != hipSuccess) return e;
+        if ((e = k2_fill_u32(st, d_err, 1, 0)) != hipSuccess) return e;
+        if ((e = k2_fill_u64(st, d_exec, 1)) != hipSuccess) return e;
+        if ((e = k2_fill_u64(st, d_sols, ns)) != hipSuccess) return e;
+        return k2_fill_u64(st, d_nodes, ns);
+    };
+    NQM_HIP(ctx, enqueue_clear());
+
+    std::vector<uint32_t> caps;
+    std::vector<uint8_t> final_row(h.row);
+    uint32_t count = count0;
+    float expand_ms = 0.f;
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+    for (uint32_t L = 0; L < 11; ++L) {
+        bool any = false;
+        for (uint32_t s = 0; s < ns; ++s)
+            if (final_row[s] < target[s]) any = true;
+        if (!any || count == 0 || (uint64_t)count >= want_items) break;
+        const uint64_t cap64 = (uint64_t)count * max_n;
+        if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
+        const uint32_t cap = (uint32_t)cap64;
+        if ((rc = ensure_dev(ctx, b.q[L & 1], (size_t)cap * sizeof(Item))) != NQM_OK) return rc;
+        const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
+        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, cap,
+                               d_target, d_nodes, d_exec, d_err));
+        uint32_t next = 0;
+        NQM_HIP(ctx, hipMemcpyAsync(&next, d_cnt + L + 1, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        caps.push_back(cap);
+        for (uint32_t s = 0; s < ns; ++s)
+            if (final_row[s] < target[s]) final_row[s]++;
+        count = next;
+    }
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+    NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+    NQM_HIP(ctx, hipEventElapsedTime(&expand_ms, ctx->ev_start, ctx->ev_stop));
+    const uint32_t levels = (uint32_t)caps.size();
+    Item* const final_q = level_in(levels);
+    const int spare = levels == 0 ? 0 : (int)(levels & 1);  // q[] not holding final_q
+
+    // ---- drain shape -------------------------------------------------------
+    uint32_t stack_rows = 1;
+    for (uint32_t s = 0; s < ns; ++s) {
+        const uint32_t n = solves[s].n;
+        if (n > final_row[s] + 1) stack_rows = std::max(stack_rows, n - 1 - final_row[s]);
+    }
+    uint32_t blocks = 0, threads = 0;
+    if ((rc = drain_shape(ctx, stack_rows, t, &blocks, &threads)) != NQM_OK) return rc;
+    const uint64_t spill_per_round = (uint64_t)blocks * threads * (stack_rows + 1u);
+
+    K2DrainParams p;
+    p.solve_sols = d_sols;
+    p.solve_nodes = d_nodes;
+    p.exec_nodes = d_exec;
+    p.err = d_err;
+    p.stack_rows = stack_rows;
+    p.refill_idle = refill;
+    p.donate = (t.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
+    p.budget = t.budget;
+    p.head = d_head;
+    p.spill_count = d_spill_count;
+    p.item_count = d_qcount;
+
+    float run_ms = 0.f;
+    uint32_t launches = levels, rounds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+
+    if (use_graph) {
+        // One graph: clears, every expand level, the drain. Replays the whole
+        // batch from the initial items with the level sizes learned above.
+        p.items = final_q;
+        p.spill = nullptr;
+        p.spill_cap = 0;
+        NQM_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_clear();
+        for (uint32_t L = 0; e == hipSuccess && L < levels; ++L) {
+            const uint32_t grid = std::min<uint32_t>(4096u, (caps[L] / max_n + 255u) / 256u);
+            e = k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, caps[L],
+                          d_target, d_nodes, d_exec, d_err);
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess) e = k2_fill_u32(st, d_head, 1, 0);
+        if (e == hipSuccess) e = k2_drain(st, blocks, threads, p);
+        hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(ctx, NQM_EDEVICE, "graph capture failed: %s / %s", hipGetErrorString(e),
+                        hipGetErrorString(e2));
+        }
+        hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(ctx, NQM_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        NQM_HIP(ctx, hipGraphLaunch(gexec, st));
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        launches = 1;
+        rounds = 1;
+    } else {
+        NQM_HIP(ctx, hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st));
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        Item* queue = final_q;
+        int spill_buf = spare;
+        uint32_t qn = count;
+        for (;;) {
+            NQM_HIP(ctx, k2_fill_u32(st, d_head, 1, 0));
+            NQM_HIP(ctx, k2_fill_u32(st, d_spill_count, 1, 0));
+            p.items = queue;
+            if (t.budget) {
+                const uint64_t cap64 = (uint64_t)qn + spill_per_round;
+                if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "spill queue too large");
+                if ((rc = ensure_dev(ctx, b.q[spill_buf], (size_t)cap64 * sizeof(Item))) != NQM_OK)
+                    return rc;
+                p.spill = static_cast<Item*>(b.q[spill_buf].ptr);
+                p.spill_cap = (uint32_t)cap64;
+            }
+            NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
+            ++launches;
+            ++rounds;
+            if (!t.budget) break;
+            NQM_HIP(ctx, k2_append_leftover(st, queue, d_qcount, d_head, p.spill, d_spill_count,
+                                            p.spill_cap, d_err));
+            uint32_t left = 0;
+            NQM_HIP(ctx, hipMemcpyAsync(&left, d_spill_count, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            if (left == 0) break;
+            NQM_HIP(ctx, hipMemcpyAsync(d_qcount, d_spill_count, 4, hipMemcpyDeviceToDevice, st));
+            queue = p.spill;
+            qn = left;
+            spill_buf ^= 1;
+            if (rounds > 100000u) return fail(ctx, NQM_ERANGE, "restart rounds do not converge");
+        }
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+    }
+
+    std::vector<uint64_t> hs(ns), hn(ns);
+    uint64_t exec = 0;
+    uint32_t err = 0;
+    NQM_HIP(ctx, hipMemcpyAsync(hs.data(), d_sols, ns * 8, hipMemcpyDeviceToHost, st));
+    NQM_HIP(ctx, hipMemcpyAsync(hn.data(), d_nodes, ns * 8, hipMemcpyDeviceToHost, st));
+    NQM_HIP(ctx, hipMemcpyAsync(&exec, d_exec, 8, hipMemcpyDeviceToHost, st));
+    NQM_HIP(ctx, hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, st));
+    NQM_HIP(ctx, hipStreamSynchronize(st));
+    NQM_HIP(ctx, hipEventElapsedTime(&run_ms, ctx->ev_start, ctx->ev_stop));
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (err) return fail(ctx, NQM_ERANGE, "queue capacity exceeded (err=0x%x)", err);
+
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+        const uint64_t c = hs[s] + h.sols[s];
+        counts[s] = c;
+        if (nodes_out) nodes_out[s] = hn[s] + h.nodes[s];
+        total += c;
+    }
+    if (stats) {
+        stats->total = total;
+        stats->nodes = exec + h.exec_nodes;
+        stats->kernel_ms = use_graph ? (double)run_ms : (double)expand_ms + (double)run_ms;
+        stats->wall_ms = now_ms() - t0;
+        stats->launches = launches;
+        stats->blocks = blocks;
+    }
+    (void)rounds;
+    return NQM_OK;
+}
+
+}  // namespace nqm
+
+extern "C" int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
+                               const nqm_tune* tune, uint64_t* counts, uint64_t* nodes,
+                               nqm_stats* stats) {
+    if (!ctx) return NQM_EINVAL;
+    if (!solves || nsolves == 0 || !counts) return nqm::fail(ctx, NQM_EINVAL, "empty batch");
+    NQM_HIP(ctx, hipSetDevice(ctx->device));
+    return nqm::solve_batch_impl(ctx, solves, nsolves, tune, counts, nodes, stats);
+}

@@ -53,11 +53,12 @@ int fail(nqm_ctx* ctx, int status, const char* fmt, ...);
 
 int ensure_dev(nqm_ctx* ctx, DevBuf& b, size_t bytes);
 int ensure_host(nqm_ctx* ctx, HostBuf& b, size_t bytes);
+double now_ms();
 
 // Largest n this build counts (u32 bitsets), as deffinitions.cuh:116-118.
 constexpr uint32_t kMaxN = 31;
 
-// Static-pool counting kernel launcher (K1). Partials: one u64 per block.
+// ---- K1: static pool (k1_static.hip) --------------------------------------
 struct K1Launch {
     uint32_t blocks = 0;
     uint32_t threads = 0;
@@ -68,5 +69,35 @@ hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_
                      uint64_t count, uint32_t n, uint32_t depth, uint32_t flags,
                      uint64_t* d_block_sols, uint64_t* d_block_nodes, uint64_t* d_per_sols,
                      uint64_t* d_per_nodes, uint32_t* d_err);
+
+// ---- K2: persistent queue (k2_queue.hip) ----------------------------------
+struct Item;
+struct K2DrainParams {
+    const Item* items = nullptr;
+    const uint32_t* item_count = nullptr;
+    uint32_t* head = nullptr;
+    Item* spill = nullptr;
+    uint32_t* spill_count = nullptr;
+    uint32_t spill_cap = 0;
+    uint64_t* solve_sols = nullptr;
+    uint64_t* solve_nodes = nullptr;
+    uint64_t* exec_nodes = nullptr;
+    uint32_t* err = nullptr;
+    uint32_t stack_rows = 1;
+    uint32_t refill_idle = 16;
+    uint32_t donate = 1;
+    uint64_t budget = 0;
+};
+hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
+hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
+hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
+                     Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
+                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err);
+hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,
+                              const uint32_t* head, Item* out, uint32_t* out_count, uint32_t out_cap,
+                              uint32_t* err);
+hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
+size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
+int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
 
 }  // namespace nqm

@@ -56,6 +56,41 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Solve(ctypes.Structure):
+    """nqm_solve: board size, pre-placed queens and their columns."""
+    _fields_ = [("n", ctypes.c_uint32), ("k", ctypes.c_uint32), ("prefix", ctypes.c_uint8 * 32)]
+
+
+class Tune(ctypes.Structure):
+    """nqm_tune (zero = automatic for every field)."""
+    _fields_ = [("depth", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("threads", ctypes.c_uint32), ("refill_idle", ctypes.c_uint32),
+                ("budget", ctypes.c_uint64), ("flags", ctypes.c_uint32),
+                ("stack_rows", ctypes.c_uint32)]
+
+
+TUNE_NO_MIRROR = 0x1
+TUNE_NO_DONATE = 0x2
+TUNE_GRAPH = 0x4
+
+
+def make_solves(solves):
+    arr = (Solve * len(solves))()
+    for i, (n, prefix) in enumerate(solves):
+        arr[i].n = n
+        arr[i].k = len(prefix)
+        for r, c in enumerate(prefix):
+            arr[i].prefix[r] = c
+    return arr
+
+
+def make_tune(**kw):
+    t = Tune()
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
 class GpuConfig(ctypes.Structure):
     """gpu_config_t of the reference (GPU_NQueens.cuh:8-11)."""
     _fields_ = [("device_id", ctypes.c_uint), ("async_", ctypes.c_int)]
@@ -78,6 +113,8 @@ _SIGNATURES = {
     "nqm_solver_driver": ([ctypes.c_uint32, VP, ctypes.c_uint32, ctypes.c_uint,
                            ctypes.POINTER(GpuConfig), ctypes.c_uint, U64P], ctypes.c_int),
     "nqm_state_size": ([ctypes.c_uint32], ctypes.c_size_t),
+    "nqm_solve_batch": ([VP, ctypes.POINTER(Solve), ctypes.c_uint32, ctypes.POINTER(Tune), U64P,
+                         U64P, ctypes.POINTER(Stats)], ctypes.c_int),
     "nqm_dev_alloc": ([VP, ctypes.c_size_t, ctypes.POINTER(VP)], ctypes.c_int),
     "nqm_dev_free": ([VP, VP], ctypes.c_int),
     "nqm_memcpy_h2d": ([VP, VP, VP, ctypes.c_size_t], ctypes.c_int),
@@ -233,6 +270,24 @@ class Context:
                 ps.free()
             if pn:
                 pn.free()
+
+
+def _solve_batch(ctx, solves, tune=None, raw=False, **tune_kw):
+    arr = make_solves(solves)
+    t = tune if tune is not None else (make_tune(**tune_kw) if tune_kw else None)
+    counts = np.zeros(len(solves), np.uint64)
+    nodes = np.zeros(len(solves), np.uint64)
+    st = Stats()
+    rc = ctx.lib.nqm_solve_batch(ctx.handle, arr, len(solves), ctypes.byref(t) if t else None,
+                                 counts.ctypes.data_as(U64P), nodes.ctypes.data_as(U64P),
+                                 ctypes.byref(st))
+    if raw:
+        return rc, counts, nodes, st
+    ctx._check(rc)
+    return counts, nodes, st
+
+
+Context.solve_batch = _solve_batch
 
 
 def states_from_pool(n, lock, pool):

@@ -99,6 +99,41 @@ int nqm_solver_driver(uint32_t n, const void* states, uint32_t state_cnt,
 /* sizeof(nq_state_t) of the reference for board size n. */
 size_t nqm_state_size(uint32_t n);
 
+/* ---- batched persistent counting ----------------------------------------
+ * Many independent solves (any mix of n and pre-placed prefixes) in one
+ * persistent device pass. No reference counterpart: the reference counts one
+ * pool of one N per launch (GPU_NQueens.cu:128-138); its README names
+ * restarts as future work (README.md:46-56). Counts are exactly those of the
+ * static path. */
+typedef struct {
+    uint32_t n;          /* board size, 1..31 */
+    uint32_t k;          /* queens pre-placed in rows 0..k-1 (0..n) */
+    uint8_t prefix[32];  /* their columns */
+} nqm_solve;
+
+#define NQM_TUNE_NO_MIRROR 0x1u  /* do not halve empty boards by mirror symmetry */
+#define NQM_TUNE_NO_DONATE 0x2u  /* no in-wave work donation (queue refill only) */
+#define NQM_TUNE_GRAPH 0x4u      /* run expand + drain as one captured HIP graph */
+
+typedef struct {
+    uint32_t depth;        /* rows placed before the queue (per solve, capped
+                              at n-1 and raised to k); 0 = automatic */
+    uint32_t waves_per_cu; /* target resident waves per CU; 0 = as many as fit */
+    uint32_t threads;      /* workgroup size 64/128/256; 0 = 256 */
+    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 16 */
+    uint64_t budget;       /* passes per wave before live work is spilled and the
+                              queue relaunched (restart); 0 = drain in one launch */
+    uint32_t flags;        /* NQM_TUNE_* */
+    uint32_t stack_rows;   /* LDS stack rows per lane; 0 = what the deepest solve needs */
+} nqm_tune;
+
+/* Count every solve. counts[i] = solutions extending solve i's prefix;
+ * nodes[i] (optional) = placements below the prefix, full-tree numbers
+ * (mirror-halved work is weighted back). stats->nodes = placements
+ * actually executed on the device. */
+int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
+                    const nqm_tune* tune, uint64_t* counts, uint64_t* nodes, nqm_stats* stats);
+
 /* ---- device memory helpers (so callers need no HIP headers) ------------ */
 int nqm_dev_alloc(nqm_ctx* ctx, size_t bytes, void** d_ptr);
 int nqm_dev_free(nqm_ctx* ctx, void* d_ptr);

@@ -12,6 +12,8 @@ Two kinds of data live here:
       n12_d3_prefixes.npz -- all 756 depth-3 prefixes of N=12 with each
                              prefix's solution and node count
       n16_d5_sample.npz   -- 512 seeded depth-5 prefixes of N=16 with counts
+      c3_batch.json       -- the 512-solve mixed batch (seeded, splitmix64):
+                             each solve's n, prefix, count and nodes below it
 
 Run from the repo root:  python tests/golden/gen_golden.py
 """
@@ -23,7 +25,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "gpu-n-queens_amd"))
 import oracle  # noqa: E402
+from nqm import workloads  # noqa: E402
 
 KNOWN = [1, 0, 0, 2, 10, 4, 40, 92, 352, 724, 2680, 14200, 73712, 365596, 2279184, 14772512,
          95815104, 666090624, 4968057848, 39029188884, 314666222712, 2691008701644,
@@ -37,6 +41,62 @@ POOLS = [(10, 7, 6358), (11, 7, 30040), (12, 8, 152604), (13, 9, 802192), (14, 1
 
 NODES = {8: 2056, 9: 8393, 10: 35538, 11: 166925, 12: 856188, 13: 4674889, 14: 27358552,
          15: 171129071, 16: 1141190302, 17: 8017021931, 18: 59365844490}
+
+
+def extensions(n, prefix, depth):
+    """Valid extensions of `prefix` to `depth` rows, and the number of
+    placements made on the way (rows len(prefix)..depth-1)."""
+    mask = (1 << n) - 1
+    cols = ld = rd = 0
+    for c in prefix:
+        b = 1 << c
+        cols |= b
+        ld = ((ld | b) << 1) & mask
+        rd = (rd | b) >> 1
+    out, placed = [], [0]
+
+    def rec(row, cols, ld, rd, pre):
+        if row == depth:
+            out.append(pre)
+            return
+        f = ~(cols | ld | rd) & mask
+        while f:
+            b = f & -f
+            f ^= b
+            placed[0] += 1
+            rec(row + 1, cols | b, ((ld | b) << 1) & mask, (rd | b) >> 1, pre + [b.bit_length() - 1])
+
+    rec(len(prefix), cols, ld, rd, list(prefix))
+    return out, placed[0]
+
+
+def oracle_solve(n, prefix):
+    """(solutions, nodes below prefix) with the threaded oracle pool counter."""
+    k = len(prefix)
+    if k == n:
+        return 1, 0
+    d = min(n - 1, k + 3)
+    if d <= k:
+        return oracle.count_prefix(n, list(prefix))
+    pool, placed = extensions(n, prefix, d)
+    if not pool:
+        return 0, placed
+    tot, nd = oracle.count_pool(n, d, np.array(pool, np.uint8))
+    return tot, nd + placed
+
+
+def c3_fixture():
+    solves = workloads.mixed_batch()
+    cache = {}
+    rows = []
+    for n, pre in solves:
+        key = (n, pre)
+        if key not in cache:
+            cache[key] = oracle_solve(n, pre)
+        c, nd = cache[key]
+        rows.append({"n": n, "prefix": list(pre), "count": c, "nodes": nd})
+    return {"seed": "0x5EED0512", "generator": "nqm.workloads.mixed_batch (splitmix64)",
+            "solves": rows}
 
 
 def main():
@@ -63,6 +123,8 @@ def main():
     _, _, ps, pn = oracle.count_pool(16, 5, sample, per_prefix=True)
     np.savez_compressed(os.path.join(HERE, "n16_d5_sample.npz"), pool=sample, sols=ps, nodes=pn,
                         index=idx)
+    with open(os.path.join(HERE, "c3_batch.json"), "w") as f:
+        json.dump(c3_fixture(), f, indent=0)
     print("golden fixtures written")
 
 
