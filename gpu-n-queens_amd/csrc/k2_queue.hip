@@ -28,6 +28,8 @@
 //    queue as independent items, and the wave exits. The host relaunches on
 //    the spill queue (restart), which also lets new work join between rounds.
 
+#include <algorithm>
+
 #include "nqm_internal.h"
 #include "nqm_items.h"
 
@@ -61,7 +63,8 @@ __global__ void __launch_bounds__(256)
 k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
                  Item* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
                  const uint8_t* __restrict__ target_row, uint64_t* __restrict__ solve_nodes,
-                 uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err) {
+                 uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err,
+                 const float* __restrict__ pred, float thresh) {
     const uint32_t cnt = *in_count;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t step = gridDim.x * blockDim.x;
@@ -75,7 +78,9 @@ k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_co
             n = meta_n(it.meta);
             row = meta_row(it.meta);
             mask = (1u << n) - 1u;
-            expand = row < target_row[it.solve];
+            // by prediction (split the large subtrees, never the last rows)
+            // or up to the solve's target row
+            expand = pred ? (pred[i] > thresh && row + 2 < n) : (row < target_row[it.solve]);
             if (expand) {
                 uint32_t f = it.f;
                 placed = __popc(f);
@@ -364,9 +369,65 @@ hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n) {
 
 hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
                      Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
-                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err) {
+                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err, const float* pred,
+                     float thresh) {
     hipLaunchKernelGGL(k2_expand_kernel, dim3(grid), dim3(256), 0, s, in, in_count, out, out_count,
-                       out_cap, target_row, solve_nodes, exec_nodes, err);
+                       out_cap, target_row, solve_nodes, exec_nodes, err, pred, thresh);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// largest-predicted-first order: counting sort on 64 buckets of half a log2
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBuckets = 64;
+
+__device__ __forceinline__ uint32_t pred_bucket(float p) {
+    // bucket 0 holds the largest predictions (log2 sizes 31.5 and up)
+    const float b = (32.0f - p) * 2.0f;
+    return b <= 0.f ? 0u : (b >= (float)(kBuckets - 1) ? kBuckets - 1 : (uint32_t)b);
+}
+
+__global__ void k2_bucket_hist_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count,
+                                      uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kBuckets];
+    for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint32_t c = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        atomicAdd(&h[pred_bucket(pred[i])], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+__global__ void k2_bucket_scan_kernel(uint32_t* __restrict__ hist) {
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i < kBuckets; ++i) {
+            const uint32_t v = hist[i];
+            hist[i] = acc;
+            acc += v;
+        }
+    }
+}
+
+__global__ void k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ pred,
+                                         const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor,
+                                         Item* __restrict__ out) {
+    const uint32_t c = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
+        const uint32_t o = atomicAdd(&cursor[pred_bucket(pred[i])], 1u);
+        out[o] = in[i];
+    }
+}
+
+hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
+                                  uint32_t max_count, uint32_t* hist, Item* out) {
+    const uint32_t blocks = std::max(1u, std::min(2048u, (max_count + 255u) / 256u));
+    hipLaunchKernelGGL(k2_fill_u32_kernel, dim3(1), dim3(64), 0, s, hist, kBuckets, 0u);
+    hipLaunchKernelGGL(k2_bucket_hist_kernel, dim3(blocks), dim3(256), 0, s, pred, count, hist);
+    hipLaunchKernelGGL(k2_bucket_scan_kernel, dim3(1), dim3(64), 0, s, hist);
+    hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, hist, out);
     return hipGetLastError();
 }
 

@@ -128,6 +128,13 @@ static int count_device_impl(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uin
     return NQM_OK;
 }
 
+// A plain streaming copy: the reference point for the FETCH_SIZE / WRITE_SIZE
+// counters (each byte is read once and written once).
+__global__ void dev_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 static int check_count_args(nqm_ctx* ctx, uint32_t n, uint32_t depth, uint64_t count,
                             uint32_t flags) {
     if (!ctx) return NQM_EINVAL;
@@ -297,6 +304,21 @@ int nqm_dev_free(nqm_ctx* ctx, void* d_ptr) {
     if (!d_ptr) return NQM_OK;
     NQM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     NQM_HIP(ctx, hipFree(d_ptr));
+    return NQM_OK;
+}
+
+int nqm_dev_copy(nqm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, double* kernel_ms) {
+    if (!ctx || (bytes && (!d_dst || !d_src)) || (bytes % 16)) return NQM_EINVAL;
+    NQM_HIP(ctx, hipSetDevice(ctx->device));
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    hipLaunchKernelGGL(dev_copy_kernel, dim3(4096), dim3(256), 0, ctx->stream, static_cast<const uint4*>(d_src),
+                       static_cast<uint4*>(d_dst), bytes / 16);
+    NQM_HIP(ctx, hipGetLastError());
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+    NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.f;
+    NQM_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (kernel_ms) *kernel_ms = ms;
     return NQM_OK;
 }
 

@@ -14,6 +14,7 @@
 // level sizes learned by a first, uncaptured expansion) and launched as one
 // graph; counters are cleared by fill kernels, never memset nodes.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -143,6 +144,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 16u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
     const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
+    if ((t.flags & NQM_TUNE_GRAPH) && t.model)
+        return fail(ctx, NQM_EUNSUPPORTED, "learned splitting decides on the host: no graph capture");
 
     HostInit h;
     int rc = build_host_items(ctx, solves, ns, mirror, &h);
@@ -169,13 +172,14 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     Batch b;
     const uint32_t count0 = (uint32_t)h.items.size();
     if ((rc = ensure_dev(ctx, b.init, std::max<size_t>(count0, 1) * sizeof(Item))) != NQM_OK) return rc;
-    for (DevBuf* d : {&b.counts, &b.head, &b.spill_count, &b.err, &b.exec})
+    for (DevBuf* d : {&b.head, &b.spill_count, &b.err, &b.exec})
         if ((rc = ensure_dev(ctx, *d, 64)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.counts, 32 * 4)) != NQM_OK) return rc;  // 32 count slots
     if ((rc = ensure_dev(ctx, b.target, std::max<size_t>(ns, 1))) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.sols, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.nodes, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
 
-    uint32_t* const d_cnt = static_cast<uint32_t*>(b.counts.ptr);  // 16 slots
+    uint32_t* const d_cnt = static_cast<uint32_t*>(b.counts.ptr);  // 32 slots
     uint32_t* const d_head = static_cast<uint32_t*>(b.head.ptr);
     uint32_t* const d_spill_count = static_cast<uint32_t*>(b.spill_count.ptr);
     uint32_t* const d_err = static_cast<uint32_t*>(b.err.ptr);
@@ -238,8 +242,78 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
     NQM_HIP(ctx, hipEventElapsedTime(&expand_ms, ctx->ev_start, ctx->ev_stop));
     const uint32_t levels = (uint32_t)caps.size();
-    Item* const final_q = level_in(levels);
-    const int spare = levels == 0 ? 0 : (int)(levels & 1);  // q[] not holding final_q
+    Item* final_q = level_in(levels);
+    int spare = levels == 0 ? 0 : (int)(levels & 1);  // q[] not holding final_q
+    uint32_t* final_slot = d_cnt + levels;
+
+    // ---- learned splitting: split predicted-large items, order largest first
+    float learn_ms = 0.f;
+    uint32_t split_levels = 0;
+    if (t.model && count) {
+        const uint32_t rounds = t.split_rounds ? t.split_rounds : 2u;
+        const uint32_t lanes = (uint32_t)ctx->prop.multiProcessorCount * 16u * 64u;
+        DevBuf feats, pred, hist;
+        struct Free {
+            DevBuf* b[3];
+            ~Free() {
+                for (DevBuf* d : b)
+                    if (d->ptr) (void)hipFree(d->ptr);
+            }
+        } fr{{&feats, &pred, &hist}};
+        if ((rc = ensure_dev(ctx, hist, 64 * 4)) != NQM_OK) return rc;
+        uint32_t* const d_slot = d_cnt + 16;  // slots 16..31 for this stage
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        auto predict = [&](Item* items, uint32_t* cnt_slot, uint32_t n_items) -> int {
+            int r;
+            if ((r = ensure_dev(ctx, feats, (size_t)n_items * 32)) != NQM_OK) return r;
+            if ((r = ensure_dev(ctx, pred, (size_t)n_items * 4)) != NQM_OK) return r;
+            NQM_HIP(ctx, mlp_features_items(st, items, cnt_slot, feats.ptr, n_items));
+            return mlp_predict_device(t.model, feats.ptr, n_items, static_cast<float*>(pred.ptr));
+        };
+        for (uint32_t r = 0; r <= rounds; ++r) {
+            if ((rc = predict(final_q, final_slot, count)) != NQM_OK) return rc;
+            if (r == rounds) break;
+            float thresh = t.split_log2;
+            std::vector<float> hp(count);
+            NQM_HIP(ctx, hipMemcpyAsync(hp.data(), pred.ptr, (size_t)count * 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            if (thresh <= 0.f) {  // split what exceeds 1/8 of a lane's fair share
+                double sum = 0;
+                for (float v : hp) sum += std::exp2((double)v);
+                thresh = (float)std::log2(std::max(1.0, sum / (8.0 * lanes)));
+            }
+            uint64_t over = 0;
+            for (float v : hp) over += v > thresh;
+            if (over == 0) break;
+            const uint64_t cap64 = (uint64_t)count + over * max_n;
+            if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
+            if ((rc = ensure_dev(ctx, b.q[spare], (size_t)cap64 * sizeof(Item))) != NQM_OK) return rc;
+            uint32_t* out_slot = d_slot + (r & 7);
+            NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
+            const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
+            NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, static_cast<Item*>(b.q[spare].ptr), out_slot,
+                                   (uint32_t)cap64, d_target, d_nodes, d_exec, d_err,
+                                   static_cast<const float*>(pred.ptr), thresh));
+            NQM_HIP(ctx, hipMemcpyAsync(&count, out_slot, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            final_q = static_cast<Item*>(b.q[spare].ptr);
+            final_slot = out_slot;
+            spare ^= 1;
+            ++split_levels;
+        }
+        if (!(t.flags & NQM_TUNE_NO_ORDER)) {
+            if ((rc = ensure_dev(ctx, b.q[spare], (size_t)std::max(count, 1u) * sizeof(Item))) != NQM_OK)
+                return rc;
+            NQM_HIP(ctx, k2_order_by_prediction(st, final_q, static_cast<const float*>(pred.ptr), final_slot,
+                                                count, static_cast<uint32_t*>(hist.ptr),
+                                                static_cast<Item*>(b.q[spare].ptr)));
+            final_q = static_cast<Item*>(b.q[spare].ptr);
+            spare ^= 1;
+        }
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+        NQM_HIP(ctx, hipEventElapsedTime(&learn_ms, ctx->ev_start, ctx->ev_stop));
+    }
 
     // ---- drain shape -------------------------------------------------------
     uint32_t stack_rows = 1;
@@ -303,7 +377,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         launches = 1;
         rounds = 1;
     } else {
-        NQM_HIP(ctx, hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st));
+        NQM_HIP(ctx, hipMemcpyAsync(d_qcount, final_slot, 4, hipMemcpyDeviceToDevice, st));
         NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
         Item* queue = final_q;
         int spill_buf = spare;
@@ -362,12 +436,13 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     if (stats) {
         stats->total = total;
         stats->nodes = exec + h.exec_nodes;
-        stats->kernel_ms = use_graph ? (double)run_ms : (double)expand_ms + (double)run_ms;
+        stats->kernel_ms = use_graph ? (double)run_ms : (double)expand_ms + (double)learn_ms + (double)run_ms;
         stats->wall_ms = now_ms() - t0;
         stats->launches = launches;
         stats->blocks = blocks;
     }
     (void)rounds;
+    (void)split_levels;
     return NQM_OK;
 }
 

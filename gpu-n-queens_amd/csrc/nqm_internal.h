@@ -92,12 +92,20 @@ hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
 hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
                      Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
-                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err);
+                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
+                     const float* pred = nullptr, float thresh = 0.f);
+hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
+                                  uint32_t max_count, uint32_t* hist, Item* out);
 hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,
                               const uint32_t* head, Item* out, uint32_t* out_count, uint32_t out_cap,
                               uint32_t* err);
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
+
+// ---- subtree-size predictor (mlp.hip) -------------------------------------
+int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y);
+hipError_t mlp_features_items(hipStream_t s, const Item* items, const uint32_t* count, void* feats,
+                              uint32_t max_count);
 
 }  // namespace nqm

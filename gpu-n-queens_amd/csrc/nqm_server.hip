@@ -1,0 +1,410 @@
+// nqm_server.hip -- count requests served by continuous batching.
+//
+// No reference counterpart (the reference counts one pool per process run).
+// Requests of any n and prefix join a device-resident work queue between
+// drain rounds; every round is one budgeted K2 drain over the whole queue,
+// so work of requests that arrived at different times overlaps on the
+// device, and a request's result is ready the round its last item ends.
+//
+//   submit: admit while a counter slot is free and the request's expanded
+//           items fit the queue memory; the request is expanded on the
+//           device (staging buffer) and appended to the queue tail.
+//   step:   one drain round (pass budget) -> unfinished work is spilled to
+//           the other queue buffer with the never-started items; a
+//           histogram of items per slot says which requests are done.
+//   poll:   finished results (count, nodes, admission and completion clock).
+// Counts per request are exactly those of nqm_solve_batch on that request
+// alone: items are disjoint subtrees, and counters are per slot.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "nqm_internal.h"
+#include "nqm_items.h"
+
+namespace nqm {
+
+__global__ void srv_zero_slots_kernel(const uint32_t* __restrict__ slots, uint32_t n, uint64_t* __restrict__ a,
+                                      uint64_t* __restrict__ b) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        a[slots[i]] = 0;
+        b[slots[i]] = 0;
+    }
+}
+
+__global__ void srv_outstanding_kernel(const Item* __restrict__ items, const uint32_t* __restrict__ count,
+                                       uint32_t* __restrict__ per_slot) {
+    const uint32_t c = *count;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        atomicAdd(&per_slot[items[i].solve], 1u);
+}
+
+}  // namespace nqm
+
+struct nqm_server {
+    nqm_ctx* ctx = nullptr;
+    nqm_server_cfg cfg{};
+    uint32_t slots_n = 0;
+    uint64_t cap_items = 0;   // admission limit of queued items
+    uint64_t buf_items = 0;   // physical capacity of each queue buffer
+    nqm::DevBuf q[2], stage[2], counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list;
+    int cur = 0;
+    uint32_t qcount = 0;
+    struct Slot {
+        bool active = false;
+        uint64_t tag = 0;
+        uint32_t n = 0, row = 0;
+        uint64_t host_sols = 0, host_nodes = 0;
+        uint32_t rounds = 0;
+        double admit_ms = 0;
+    };
+    std::vector<Slot> slots;
+    std::vector<uint32_t> free_slots;
+    std::vector<nqm_result> done;
+    std::chrono::steady_clock::time_point t0;
+    uint32_t active = 0;
+    uint64_t rounds_total = 0;
+    ~nqm_server() {
+        for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &counts, &head, &spill_count, &err, &target,
+                               &sols, &nodes, &exec, &outst, &slot_list})
+            if (d->ptr) (void)hipFree(d->ptr);
+    }
+    double clock_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace nqm {
+
+namespace {
+
+// Expand one request's initial items (host) into the staging buffer, up to
+// its target row. Returns the item count left in stage[*which].
+int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint32_t* items_out, int* which) {
+    nqm_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    int rc;
+    const uint32_t n = q.n;
+    const uint32_t mask = (1u << n) - 1u;
+    nqm_server::Slot& sl = s->slots[slot];
+    // initial items (as nqm_solve_batch builds them)
+    std::vector<Item> init;
+    uint32_t cols = 0, ld = 0, rd = 0;
+    for (uint32_t i = 0; i < q.k; ++i) {
+        const uint32_t b = 1u << q.prefix[i];
+        cols |= b;
+        ld = ((ld | b) << 1) & mask;
+        rd = (rd | b) >> 1;
+    }
+    uint32_t row = q.k;
+    const bool mirror = !(s->cfg.flags & NQM_TUNE_NO_MIRROR);
+    if (q.k == q.n) {
+        sl.host_sols = 1;
+    } else if (q.k == 0 && mirror && n >= 2) {
+        row = 1;
+        for (uint32_t c = 0; c < (n + 1) / 2; ++c) {
+            const uint32_t ws = c < n / 2 ? 1u : 0u;
+            const uint32_t b = 1u << c;
+            Item it{};
+            it.cols = b;
+            it.ld = (b << 1) & mask;
+            it.rd = b >> 1;
+            it.f = ~(it.cols | it.ld | it.rd) & mask;
+            it.solve = slot;
+            it.meta = item_meta(n, 1, ws);
+            sl.host_nodes += 1ull << ws;
+            if (it.f) init.push_back(it);
+        }
+    } else {
+        Item it{};
+        it.cols = cols;
+        it.ld = ld;
+        it.rd = rd;
+        it.f = ~(cols | ld | rd) & mask;
+        it.solve = slot;
+        it.meta = item_meta(n, q.k, 0);
+        if (it.f) init.push_back(it);
+    }
+    const uint32_t depth = s->cfg.depth ? s->cfg.depth : 6u;
+    const uint32_t target = std::min(n - 1, std::max(row, depth));
+    sl.row = row < target && !init.empty() ? target : row;
+    uint8_t tgt[1] = {(uint8_t)target};
+    NQM_HIP(ctx, hipMemcpyAsync(static_cast<uint8_t*>(s->target.ptr) + slot, tgt, 1, hipMemcpyHostToDevice, st));
+    uint32_t count = (uint32_t)init.size();
+    if ((rc = ensure_dev(ctx, s->stage[0], std::max<size_t>(count, 1) * sizeof(Item))) != NQM_OK) return rc;
+    if (count)
+        NQM_HIP(ctx, hipMemcpyAsync(s->stage[0].ptr, init.data(), count * sizeof(Item), hipMemcpyHostToDevice, st));
+    uint32_t* cnt = static_cast<uint32_t*>(s->counts.ptr);
+    NQM_HIP(ctx, k2_fill_u32(st, cnt, 1, count));
+    int w = 0;
+    for (uint32_t r = row; r < target && count; ++r) {
+        const uint64_t cap = (uint64_t)count * n;
+        if (cap > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "request expansion exceeds 2^31 items");
+        if ((rc = ensure_dev(ctx, s->stage[w ^ 1], cap * sizeof(Item))) != NQM_OK) return rc;
+        NQM_HIP(ctx, k2_fill_u32(st, cnt + 1 + (w ^ 1), 1, 0));
+        const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
+        NQM_HIP(ctx, k2_expand(st, grid, static_cast<Item*>(s->stage[w].ptr), w == 0 && r == row ? cnt : cnt + 1 + w,
+                               static_cast<Item*>(s->stage[w ^ 1].ptr), cnt + 1 + (w ^ 1), (uint32_t)cap,
+                               static_cast<uint8_t*>(s->target.ptr), static_cast<uint64_t*>(s->nodes.ptr),
+                               static_cast<uint64_t*>(s->exec.ptr), static_cast<uint32_t*>(s->err.ptr)));
+        NQM_HIP(ctx, hipMemcpyAsync(&count, cnt + 1 + (w ^ 1), 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        w ^= 1;
+    }
+    *items_out = count;
+    *which = w;
+    return NQM_OK;
+}
+
+uint32_t stack_rows_needed(const nqm_server* s) {
+    uint32_t rows = 1;
+    for (const auto& sl : s->slots)
+        if (sl.active && sl.n > sl.row + 1) rows = std::max(rows, sl.n - 1 - sl.row);
+    return rows;
+}
+
+}  // namespace
+
+}  // namespace nqm
+
+using namespace nqm;
+
+extern "C" {
+
+int nqm_server_create(nqm_ctx* ctx, const nqm_server_cfg* cfg, nqm_server** out) {
+    if (!ctx || !out) return NQM_EINVAL;
+    *out = nullptr;
+    NQM_HIP(ctx, hipSetDevice(ctx->device));
+    nqm_server* s = new (std::nothrow) nqm_server();
+    if (!s) return NQM_ENOMEM;
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE)) {
+        delete s;
+        return fail(ctx, NQM_EINVAL, "unknown server flags");
+    }
+    s->slots_n = s->cfg.max_active ? std::min(s->cfg.max_active, 1u << 20) : 256u;
+    const uint64_t qbytes = s->cfg.queue_bytes ? s->cfg.queue_bytes : (1ull << 30);
+    s->cap_items = std::max<uint64_t>(qbytes / sizeof(Item), 1024);
+    // headroom for one round of spills from every lane (31 rows at most)
+    const uint64_t spill = (uint64_t)ctx->prop.multiProcessorCount * 64u * 32u * 32u;
+    s->buf_items = s->cap_items + spill;
+    if (s->buf_items > 0x7fffffffull) {
+        delete s;
+        return fail(ctx, NQM_ERANGE, "queue too large");
+    }
+    int rc = NQM_OK;
+    for (DevBuf* d : {&s->q[0], &s->q[1]})
+        if (rc == NQM_OK) rc = ensure_dev(ctx, *d, s->buf_items * sizeof(Item));
+    for (DevBuf* d : {&s->counts, &s->head, &s->spill_count, &s->err, &s->exec})
+        if (rc == NQM_OK) rc = ensure_dev(ctx, *d, 64);
+    if (rc == NQM_OK) rc = ensure_dev(ctx, s->target, s->slots_n);
+    for (DevBuf* d : {&s->sols, &s->nodes})
+        if (rc == NQM_OK) rc = ensure_dev(ctx, *d, (size_t)s->slots_n * 8);
+    if (rc == NQM_OK) rc = ensure_dev(ctx, s->outst, (size_t)s->slots_n * 4);
+    if (rc == NQM_OK) rc = ensure_dev(ctx, s->slot_list, (size_t)s->slots_n * 4);
+    if (rc != NQM_OK) {
+        delete s;
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    if (k2_fill_u32(st, static_cast<uint32_t*>(s->err.ptr), 1, 0) != hipSuccess ||
+        k2_fill_u64(st, static_cast<uint64_t*>(s->exec.ptr), 1) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        delete s;
+        return fail(ctx, NQM_EDEVICE, "server init failed");
+    }
+    s->slots.resize(s->slots_n);
+    for (uint32_t i = s->slots_n; i > 0; --i) s->free_slots.push_back(i - 1);
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return NQM_OK;
+}
+
+void nqm_server_destroy(nqm_server* s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+}
+
+double nqm_server_clock_ms(const nqm_server* s) { return s ? s->clock_ms() : 0.0; }
+
+int nqm_server_submit(nqm_server* s, const nqm_solve* reqs, const uint64_t* tags, uint32_t nreq,
+                      uint32_t* admitted) {
+    if (!s || (nreq && (!reqs || !tags)) || !admitted) return NQM_EINVAL;
+    nqm_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    *admitted = 0;
+    for (uint32_t i = 0; i < nreq; ++i) {
+        const nqm_solve& q = reqs[i];
+        if (q.n < 1 || q.n > kMaxN || q.k > q.n) return fail(ctx, NQM_EINVAL, "request %u: n=%u k=%u", i, q.n, q.k);
+        {  // validate the prefix
+            uint32_t c = 0, l = 0, r = 0;
+            const uint32_t mask = (1u << q.n) - 1u;
+            for (uint32_t j = 0; j < q.k; ++j) {
+                if (q.prefix[j] >= q.n) return fail(ctx, NQM_EPREFIX, "request %u: column out of range", i);
+                const uint32_t b = 1u << q.prefix[j];
+                if (b & (c | l | r)) return fail(ctx, NQM_EPREFIX, "request %u: invalid prefix", i);
+                c |= b;
+                l = ((l | b) << 1) & mask;
+                r = (r | b) >> 1;
+            }
+        }
+        if (s->free_slots.empty()) break;  // every counter slot is in flight
+        const uint32_t slot = s->free_slots.back();
+        nqm_server::Slot& sl = s->slots[slot];
+        sl = nqm_server::Slot();
+        sl.n = q.n;
+        sl.tag = tags[i];
+        // clear the slot's device counters before any expansion adds to them
+        uint32_t one[1] = {slot};
+        NQM_HIP(ctx, hipMemcpyAsync(s->slot_list.ptr, one, 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(srv_zero_slots_kernel, dim3(1), dim3(64), 0, st,
+                           static_cast<const uint32_t*>(s->slot_list.ptr), 1u,
+                           static_cast<uint64_t*>(s->sols.ptr), static_cast<uint64_t*>(s->nodes.ptr));
+        NQM_HIP(ctx, hipGetLastError());
+        uint32_t items = 0;
+        int which = 0;
+        int rc = stage_request(s, q, slot, &items, &which);
+        if (rc != NQM_OK) return rc;
+        if ((uint64_t)s->qcount + items > s->cap_items) break;  // no queue memory: try later
+        if (items)
+            NQM_HIP(ctx, hipMemcpyAsync(static_cast<Item*>(s->q[s->cur].ptr) + s->qcount, s->stage[which].ptr,
+                                        (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice, st));
+        s->qcount += items;
+        s->free_slots.pop_back();
+        sl.active = true;
+        sl.admit_ms = s->clock_ms();
+        ++s->active;
+        ++*admitted;
+        if (items == 0) {  // answered on admission (complete or dead prefix)
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            uint64_t nd = 0;
+            NQM_HIP(ctx, hipMemcpy(&nd, static_cast<uint64_t*>(s->nodes.ptr) + slot, 8, hipMemcpyDeviceToHost));
+            nqm_result r{};
+            r.tag = sl.tag;
+            r.count = sl.host_sols;
+            r.nodes = nd + sl.host_nodes;
+            r.rounds = 0;
+            r.admit_ms = sl.admit_ms;
+            r.done_ms = s->clock_ms();
+            s->done.push_back(r);
+            sl.active = false;
+            --s->active;
+            s->free_slots.push_back(slot);
+        }
+    }
+    NQM_HIP(ctx, hipStreamSynchronize(st));
+    return NQM_OK;
+}
+
+int nqm_server_step(nqm_server* s, uint32_t* active_out) {
+    if (!s) return NQM_EINVAL;
+    nqm_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    int rc;
+    if (s->qcount) {
+        const uint32_t rows = stack_rows_needed(s);
+        nqm_tune t{};
+        uint32_t blocks = 0, threads = 0;
+        {
+            // same grid rule as the batch path
+            uint32_t th = 256;
+            while (th > 64 && k2_lds_bytes(rows, th) > ctx->lds_per_block) th >>= 1;
+            int per_cu = 0;
+            if ((rc = k2_max_blocks_per_cu(th, k2_lds_bytes(rows, th), &per_cu)) != NQM_OK)
+                return fail(ctx, rc, "occupancy query failed");
+            blocks = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->prop.multiProcessorCount;
+            threads = th;
+        }
+        (void)t;
+        const uint64_t spill_bound = (uint64_t)blocks * threads * (rows + 1u);
+        const uint64_t budget = s->cfg.budget ? s->cfg.budget : 16384u;
+        // with no room for a full round of spills, drain this round to the end
+        const bool spill_ok = (uint64_t)s->qcount + spill_bound <= s->buf_items;
+        uint32_t* cnt = static_cast<uint32_t*>(s->counts.ptr);
+        uint32_t* qslot = cnt + 8;
+        NQM_HIP(ctx, k2_fill_u32(st, qslot, 1, s->qcount));
+        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->head.ptr), 1, 0));
+        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->spill_count.ptr), 1, 0));
+        K2DrainParams p;
+        p.items = static_cast<Item*>(s->q[s->cur].ptr);
+        p.item_count = qslot;
+        p.head = static_cast<uint32_t*>(s->head.ptr);
+        p.spill = static_cast<Item*>(s->q[s->cur ^ 1].ptr);
+        p.spill_count = static_cast<uint32_t*>(s->spill_count.ptr);
+        p.spill_cap = (uint32_t)s->buf_items;
+        p.solve_sols = static_cast<uint64_t*>(s->sols.ptr);
+        p.solve_nodes = static_cast<uint64_t*>(s->nodes.ptr);
+        p.exec_nodes = static_cast<uint64_t*>(s->exec.ptr);
+        p.err = static_cast<uint32_t*>(s->err.ptr);
+        p.stack_rows = rows;
+        p.refill_idle = 16;
+        p.donate = (s->cfg.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
+        p.budget = spill_ok ? budget : 0;
+        NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
+        NQM_HIP(ctx, k2_append_leftover(st, p.items, qslot, p.head, p.spill, p.spill_count, p.spill_cap, p.err));
+        // items still queued per slot
+        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->outst.ptr), s->slots_n, 0));
+        hipLaunchKernelGGL(srv_outstanding_kernel, dim3(1024), dim3(256), 0, st, p.spill, p.spill_count,
+                           static_cast<uint32_t*>(s->outst.ptr));
+        NQM_HIP(ctx, hipGetLastError());
+        uint32_t left = 0, err = 0;
+        NQM_HIP(ctx, hipMemcpyAsync(&left, p.spill_count, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&err, p.err, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        if (err) return fail(ctx, NQM_ERANGE, "server queue capacity exceeded (err=0x%x)", err);
+        s->qcount = left;
+        s->cur ^= 1;
+        ++s->rounds_total;
+    }
+    // completed requests: active slots with nothing left in the queue
+    if (s->active) {
+        std::vector<uint32_t> outst(s->slots_n);
+        std::vector<uint64_t> hs(s->slots_n), hn(s->slots_n);
+        NQM_HIP(ctx, hipMemcpyAsync(outst.data(), s->outst.ptr, (size_t)s->slots_n * 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(hs.data(), s->sols.ptr, (size_t)s->slots_n * 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(hn.data(), s->nodes.ptr, (size_t)s->slots_n * 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        const double now = s->clock_ms();
+        for (uint32_t i = 0; i < s->slots_n; ++i) {
+            nqm_server::Slot& sl = s->slots[i];
+            if (!sl.active) continue;
+            ++sl.rounds;
+            if (s->qcount != 0 && outst[i] != 0) continue;
+            nqm_result r{};
+            r.tag = sl.tag;
+            r.count = hs[i] + sl.host_sols;
+            r.nodes = hn[i] + sl.host_nodes;
+            r.rounds = sl.rounds;
+            r.admit_ms = sl.admit_ms;
+            r.done_ms = now;
+            s->done.push_back(r);
+            sl.active = false;
+            --s->active;
+            s->free_slots.push_back(i);
+        }
+    }
+    if (active_out) *active_out = s->active;
+    return NQM_OK;
+}
+
+int nqm_server_poll(nqm_server* s, nqm_result* out, uint32_t max, uint32_t* nout) {
+    if (!s || !nout || (max && !out)) return NQM_EINVAL;
+    const uint32_t k = (uint32_t)std::min<size_t>(max, s->done.size());
+    for (uint32_t i = 0; i < k; ++i) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *nout = k;
+    return NQM_OK;
+}
+
+int nqm_server_info(const nqm_server* s, uint32_t* active, uint64_t* queued_items, uint64_t* rounds) {
+    if (!s) return NQM_EINVAL;
+    if (active) *active = s->active;
+    if (queued_items) *queued_items = s->qcount;
+    if (rounds) *rounds = s->rounds_total;
+    return NQM_OK;
+}
+
+}  // extern "C"

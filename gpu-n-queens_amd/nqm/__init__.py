@@ -66,12 +66,14 @@ class Tune(ctypes.Structure):
     _fields_ = [("depth", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
                 ("threads", ctypes.c_uint32), ("refill_idle", ctypes.c_uint32),
                 ("budget", ctypes.c_uint64), ("flags", ctypes.c_uint32),
-                ("stack_rows", ctypes.c_uint32)]
+                ("stack_rows", ctypes.c_uint32), ("model", ctypes.c_void_p),
+                ("split_log2", ctypes.c_float), ("split_rounds", ctypes.c_uint32)]
 
 
 TUNE_NO_MIRROR = 0x1
 TUNE_NO_DONATE = 0x2
 TUNE_GRAPH = 0x4
+TUNE_NO_ORDER = 0x8
 
 
 def make_solves(solves):
@@ -87,8 +89,24 @@ def make_solves(solves):
 def make_tune(**kw):
     t = Tune()
     for k, v in kw.items():
+        if k == "model" and v is not None and not isinstance(v, int):
+            v = v.handle
         setattr(t, k, v)
     return t
+
+
+class ServerCfg(ctypes.Structure):
+    """nqm_server_cfg (zero = default for every field)."""
+    _fields_ = [("max_active", ctypes.c_uint32), ("depth", ctypes.c_uint32),
+                ("budget", ctypes.c_uint64), ("queue_bytes", ctypes.c_uint64),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    """nqm_result: one finished request."""
+    _fields_ = [("tag", ctypes.c_uint64), ("count", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
+                ("rounds", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
 
 
 class GpuConfig(ctypes.Structure):
@@ -115,9 +133,49 @@ _SIGNATURES = {
     "nqm_state_size": ([ctypes.c_uint32], ctypes.c_size_t),
     "nqm_solve_batch": ([VP, ctypes.POINTER(Solve), ctypes.c_uint32, ctypes.POINTER(Tune), U64P,
                          U64P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_pool_count": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P],
+                       ctypes.c_int),
+    "nqm_pool_generate": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U8P,
+                           ctypes.c_uint64, U64P], ctypes.c_int),
+    "nqm_pool_generate_device": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.POINTER(VP), U64P], ctypes.c_int),
+    "nqm_choose_lock": ([VP, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                         ctypes.POINTER(ctypes.c_uint32), U64P], ctypes.c_int),
+    "nqm_nqsf_write": ([ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, U8P, ctypes.c_uint64,
+                        ctypes.c_uint64], ctypes.c_int),
+    "nqm_nqsf_read": ([ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(U8P), U64P,
+                       ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), U64P],
+                      ctypes.c_int),
+    "nqm_free": ([VP], None),
+    "nqm_states_to_pool": ([ctypes.c_uint32, VP, ctypes.c_uint64, ctypes.c_uint32, U8P], ctypes.c_int),
+    "nqm_pool_to_states": ([ctypes.c_uint32, ctypes.c_uint32, U8P, ctypes.c_uint64, VP], ctypes.c_int),
+    "nqm_mlp_param_count": ([], ctypes.c_uint32),
+    "nqm_mlp_feature_width": ([], ctypes.c_uint32),
+    "nqm_mlp_create": ([VP, ctypes.c_uint64, ctypes.POINTER(VP)], ctypes.c_int),
+    "nqm_mlp_destroy": ([VP], None),
+    "nqm_mlp_set_params": ([VP, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "nqm_mlp_get_params": ([VP, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "nqm_mlp_features_pool": ([VP, ctypes.c_uint32, ctypes.c_uint32, VP, ctypes.c_uint64, VP],
+                              ctypes.c_int),
+    "nqm_mlp_predict": ([VP, VP, ctypes.c_uint32, VP], ctypes.c_int),
+    "nqm_mlp_gradient": ([VP, VP, VP, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                          ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "nqm_mlp_train": ([VP, VP, VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                       ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                       ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "nqm_server_create": ([VP, ctypes.POINTER(ServerCfg), ctypes.POINTER(VP)], ctypes.c_int),
+    "nqm_server_destroy": ([VP], None),
+    "nqm_server_submit": ([VP, ctypes.POINTER(Solve), U64P, ctypes.c_uint32,
+                           ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
+    "nqm_server_step": ([VP, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
+    "nqm_server_poll": ([VP, ctypes.POINTER(Result), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
+                        ctypes.c_int),
+    "nqm_server_info": ([VP, ctypes.POINTER(ctypes.c_uint32), U64P, U64P], ctypes.c_int),
+    "nqm_server_clock_ms": ([VP], ctypes.c_double),
     "nqm_dev_alloc": ([VP, ctypes.c_size_t, ctypes.POINTER(VP)], ctypes.c_int),
     "nqm_dev_free": ([VP, VP], ctypes.c_int),
     "nqm_memcpy_h2d": ([VP, VP, VP, ctypes.c_size_t], ctypes.c_int),
+    "nqm_dev_copy": ([VP, VP, VP, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "nqm_memcpy_d2h": ([VP, VP, VP, ctypes.c_size_t], ctypes.c_int),
 }
 
@@ -290,6 +348,110 @@ def _solve_batch(ctx, solves, tune=None, raw=False, **tune_kw):
 Context.solve_batch = _solve_batch
 
 
+def _pool_count(ctx, n, lock, flags=0, per_col=False):
+    cnt = ctypes.c_uint64()
+    pc = np.zeros(32, np.uint64)
+    ctx._check(ctx.lib.nqm_pool_count(ctx.handle, n, lock, flags, ctypes.byref(cnt),
+                                      pc.ctypes.data_as(U64P)))
+    return (cnt.value, pc[:n]) if per_col else cnt.value
+
+
+def _pool_generate(ctx, n, lock, flags=0):
+    """Host copy of the device-generated pool, shape (count, lock)."""
+    d = VP()
+    cnt = ctypes.c_uint64()
+    ctx._check(ctx.lib.nqm_pool_generate_device(ctx.handle, n, lock, flags, ctypes.byref(d),
+                                                ctypes.byref(cnt)))
+    out = np.zeros((cnt.value, lock), np.uint8)
+    try:
+        if out.size:
+            ctx._check(ctx.lib.nqm_memcpy_d2h(ctx.handle, out.ctypes.data, d.value, out.size))
+    finally:
+        ctx.lib.nqm_dev_free(ctx.handle, d.value)
+    return out
+
+
+def _pool_generate_device(ctx, n, lock, flags=0):
+    """Device-resident pool: (DeviceBuffer-like object, count)."""
+    d = VP()
+    cnt = ctypes.c_uint64()
+    ctx._check(ctx.lib.nqm_pool_generate_device(ctx.handle, n, lock, flags, ctypes.byref(d),
+                                                ctypes.byref(cnt)))
+    buf = DeviceBuffer.__new__(DeviceBuffer)
+    buf.ctx, buf.nbytes, buf.ptr = ctx, cnt.value * lock, d.value
+    return buf, cnt.value
+
+
+def _choose_lock(ctx, n, how_many, flags=MIRROR_WEIGHT):
+    lk, cnt = ctypes.c_uint32(), ctypes.c_uint64()
+    ctx._check(ctx.lib.nqm_choose_lock(ctx.handle, n, how_many, flags, ctypes.byref(lk),
+                                       ctypes.byref(cnt)))
+    return lk.value, cnt.value
+
+
+Context.pool_count = _pool_count
+Context.pool_generate = _pool_generate
+Context.pool_generate_device = _pool_generate_device
+Context.choose_lock = _choose_lock
+
+NQSF_FLAG_EXPERIMENTAL = 0x1
+NQSF_FLAG_HALF = 0x2
+NQSF_REFERENCE_FLAGS = NQSF_FLAG_EXPERIMENTAL | NQSF_FLAG_HALF
+
+
+def nqsf_write(path, n, lock, pool, flags=NQSF_REFERENCE_FLAGS):
+    a = _pool_array(pool, lock)
+    src = a if a.size else np.zeros(1, np.uint8)
+    rc = load().nqm_nqsf_write(os.fsencode(path), n, lock, src.ctypes.data_as(U8P), a.shape[0], flags)
+    if rc != OK:
+        raise NqmError(rc, f"nqm_nqsf_write({path})")
+
+
+def nqsf_read(path, expect_n=0, expect_flags=(1 << 64) - 1, raw=False):
+    """(n, lock, flags, pool) of a .nqsf file."""
+    L = load()
+    p = U8P()
+    cnt = ctypes.c_uint64()
+    n, lk = ctypes.c_uint32(), ctypes.c_uint32()
+    fl = ctypes.c_uint64()
+    rc = L.nqm_nqsf_read(os.fsencode(path), expect_n, expect_flags, ctypes.byref(p), ctypes.byref(cnt),
+                         ctypes.byref(n), ctypes.byref(lk), ctypes.byref(fl))
+    if rc != OK:
+        if raw:
+            return rc, None
+        raise NqmError(rc, f"nqm_nqsf_read({path})")
+    try:
+        pool = np.ctypeslib.as_array(p, shape=(cnt.value * lk.value,)).copy() if cnt.value * lk.value \
+            else np.zeros(0, np.uint8)
+    finally:
+        L.nqm_free(ctypes.cast(p, VP))
+    out = (n.value, lk.value, fl.value, pool.reshape(cnt.value, lk.value) if lk.value else pool)
+    return (OK, out) if raw else out
+
+
+def states_to_pool(n, states, lock):
+    L = load()
+    states = np.ascontiguousarray(states, dtype=np.uint8)
+    cnt = states.shape[0]
+    out = np.zeros((cnt, lock), np.uint8)
+    rc = L.nqm_states_to_pool(n, states.ctypes.data, cnt, lock,
+                              (out if out.size else np.zeros(1, np.uint8)).ctypes.data_as(U8P))
+    if rc != OK:
+        raise NqmError(rc, "nqm_states_to_pool")
+    return out
+
+
+def pool_to_states(n, lock, pool):
+    L = load()
+    a = _pool_array(pool, lock)
+    out = np.zeros((a.shape[0], state_size(n)), np.uint8)
+    src = a if a.size else np.zeros(1, np.uint8)
+    rc = L.nqm_pool_to_states(n, lock, src.ctypes.data_as(U8P), a.shape[0], out.ctypes.data)
+    if rc != OK:
+        raise NqmError(rc, "nqm_pool_to_states")
+    return out
+
+
 def states_from_pool(n, lock, pool):
     """Build an array in the reference's nq_state_t layout (n_queens.cuh:100-113)
     from a compact pool, as the reference's loader does (nq_utils.cu:244-251)."""
@@ -324,3 +486,139 @@ def solver_driver(n, states, locked_row_end, device=0):
     if rc != OK:
         raise NqmError(rc, "nqm_solver_driver")
     return tot.value
+
+
+# ---- subtree-size predictor ---------------------------------------------------
+
+FP = ctypes.POINTER(ctypes.c_float)
+
+
+class Mlp:
+    """nqm_mlp: 12 -> 64 -> 64 -> 1 predictor of log2(1 + subtree nodes)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = VP()
+        ctx._check(self.lib.nqm_mlp_create(ctx.handle, seed, ctypes.byref(h)))
+        self.handle = h.value
+        self.nparams = int(self.lib.nqm_mlp_param_count())
+        self.width = int(self.lib.nqm_mlp_feature_width())
+
+    def close(self):
+        if self.handle:
+            self.lib.nqm_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        self.ctx._check(self.lib.nqm_mlp_get_params(self.handle, p.ctypes.data_as(FP)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        self.ctx._check(self.lib.nqm_mlp_set_params(self.handle, p.ctypes.data_as(FP)))
+
+    def predict(self, d_feats, count):
+        out = DeviceBuffer(self.ctx, 4 * max(count, 1))
+        try:
+            self.ctx._check(self.lib.nqm_mlp_predict(self.handle, d_feats.ptr, count, out.ptr))
+            return out.to_host(np.float32, count)
+        finally:
+            out.free()
+
+    def gradient(self, d_feats, d_labels, count):
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        self.ctx._check(self.lib.nqm_mlp_gradient(self.handle, d_feats.ptr, d_labels.ptr, count,
+                                                  g.ctypes.data_as(FP), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, d_feats, d_labels, count, epochs=10, batch=4096, lr=1e-3, seed=7, val_frac=0.2):
+        tl, vl = ctypes.c_float(), ctypes.c_float()
+        self.ctx._check(self.lib.nqm_mlp_train(self.handle, d_feats.ptr, d_labels.ptr, count, epochs,
+                                               batch, lr, seed, val_frac, ctypes.byref(tl),
+                                               ctypes.byref(vl)))
+        return tl.value, vl.value
+
+
+def _features_pool(ctx, n, depth, d_pool, count):
+    """Device feature matrix (count x 16 halves) of a device pool."""
+    width = int(ctx.lib.nqm_mlp_feature_width())
+    out = DeviceBuffer(ctx, 2 * width * max(count, 1))
+    ctx._check(ctx.lib.nqm_mlp_features_pool(ctx.handle, n, depth,
+                                             d_pool.ptr if isinstance(d_pool, DeviceBuffer) else d_pool,
+                                             count, out.ptr))
+    return out
+
+
+Context.features_pool = _features_pool
+Context.mlp = lambda self, seed=1: Mlp(self, seed)
+
+
+# ---- serving ------------------------------------------------------------------
+
+class Server:
+    """nqm_server: continuous batching of count requests on one context."""
+
+    def __init__(self, ctx, **cfg):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        c = ServerCfg()
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        h = VP()
+        ctx._check(self.lib.nqm_server_create(ctx.handle, ctypes.byref(c), ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self.lib.nqm_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock_ms(self):
+        return float(self.lib.nqm_server_clock_ms(self.handle))
+
+    def submit(self, reqs, tags):
+        """Offer (n, prefix) requests in order; returns how many were admitted."""
+        if not reqs:
+            return 0
+        arr = make_solves(reqs)
+        t = np.ascontiguousarray(tags, np.uint64)
+        got = ctypes.c_uint32()
+        self.ctx._check(self.lib.nqm_server_submit(self.handle, arr, t.ctypes.data_as(U64P), len(reqs),
+                                                   ctypes.byref(got)))
+        return got.value
+
+    def step(self):
+        act = ctypes.c_uint32()
+        self.ctx._check(self.lib.nqm_server_step(self.handle, ctypes.byref(act)))
+        return act.value
+
+    def poll(self, max_results=4096):
+        buf = (Result * max_results)()
+        got = ctypes.c_uint32()
+        self.ctx._check(self.lib.nqm_server_poll(self.handle, buf, max_results, ctypes.byref(got)))
+        return [(r.tag, r.count, r.nodes, r.rounds, r.admit_ms, r.done_ms) for r in buf[:got.value]]
+
+    def info(self):
+        a, q, r = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx._check(self.lib.nqm_server_info(self.handle, ctypes.byref(a), ctypes.byref(q),
+                                                 ctypes.byref(r)))
+        return {"active": a.value, "queued_items": q.value, "rounds": r.value}
+
+
+Context.server = lambda self, **cfg: Server(self, **cfg)

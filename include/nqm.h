@@ -99,6 +99,79 @@ int nqm_solver_driver(uint32_t n, const void* states, uint32_t state_cnt,
 /* sizeof(nq_state_t) of the reference for board size n. */
 size_t nqm_state_size(uint32_t n);
 
+/* ---- prefix-pool generation (on the device) -----------------------------
+ * The reference's pool: every valid placement of rows 0..lock-1 whose row
+ * `lock` still has a free cell, in lexicographic order; with
+ * NQM_MIRROR_WEIGHT only row-0 columns < ceil(n/2) (the half pool the
+ * reference's build always generates). Replaces gen_states_rowlock and its
+ * threaded drivers (n_queens.cu:27-83, :235-315); byte-identical output.
+ * Requires 2 <= n <= 31 and 1 <= lock < n. */
+int nqm_pool_count(nqm_ctx* ctx, uint32_t n, uint32_t lock, uint32_t flags,
+                   uint64_t* count, uint64_t* per_col /* n entries, optional */);
+/* Host output: count x lock bytes; NQM_ERANGE (with *count set) if cap is short. */
+int nqm_pool_generate(nqm_ctx* ctx, uint32_t n, uint32_t lock, uint32_t flags,
+                      uint8_t* out, uint64_t cap, uint64_t* count);
+/* Device output, allocated by the library; release with nqm_dev_free. */
+int nqm_pool_generate_device(nqm_ctx* ctx, uint32_t n, uint32_t lock, uint32_t flags,
+                             uint8_t** d_out, uint64_t* count);
+/* The reference's lock-row search for a requested pool size (nq_generate_states,
+ * n_queens.cu:128-219): the lock whose pool the reference would return, and
+ * that pool's size. NQM_ERANGE where the reference would FAIL_IF. */
+int nqm_choose_lock(nqm_ctx* ctx, uint32_t n, uint64_t how_many, uint32_t flags,
+                    uint32_t* lock_out, uint64_t* count_out);
+
+/* ---- pool files and the reference's state layout (host only) ------------
+ * .nqsf, the reference's pool file (nq_utils.cu:127-259): u32 version (1),
+ * u64 breaking-flag word, u8 N, u8 lock, u64 total, then per populated row
+ * h: u8 h, u64 count, count x h column bytes. Flag word bit 0 = the
+ * reference's experimental-optimisation build flag, bit 1 = mirror-halved
+ * pool (nq_utils.cu:65-74); the reference's default build writes 3. */
+#define NQM_NQSF_FLAG_EXPERIMENTAL 0x1ull
+#define NQM_NQSF_FLAG_HALF 0x2ull
+int nqm_nqsf_write(const char* path, uint32_t n, uint32_t lock, const uint8_t* pool,
+                   uint64_t count, uint64_t flags);
+/* Reads a pool whose prefixes all have `lock` rows. expect_n = 0 skips the
+ * board-size check; expect_flags = UINT64_MAX skips the flag check (else a
+ * mismatch is NQM_EFLAGS, as util_read_nq_states_from_stream returns -4).
+ * *pool is malloc'ed by the library: release with nqm_free. */
+int nqm_nqsf_read(const char* path, uint32_t expect_n, uint64_t expect_flags,
+                  uint8_t** pool, uint64_t* count, uint32_t* n_out, uint32_t* lock_out,
+                  uint64_t* flags_out);
+void nqm_free(void* p);
+/* Reference nq_state_t array (see nqm_solver_driver) <-> compact pool. */
+int nqm_states_to_pool(uint32_t n, const void* states, uint64_t count, uint32_t lock,
+                       uint8_t* pool_out);
+int nqm_pool_to_states(uint32_t n, uint32_t lock, const uint8_t* pool, uint64_t count,
+                       void* states_out);
+
+/* ---- subtree-size predictor (learned splitting) --------------------------
+ * No reference counterpart. A 12 -> 64 -> 64 -> 1 MLP (ReLU, FP16 weights,
+ * FP32 accumulation on MFMA) estimating log2(1 + nodes below a prefix) from
+ * relative next-row bitsets. Trained on the device by the library's own
+ * forward/backward kernels (Adam on FP32 master weights). Features are
+ * nqm_mlp_feature_width() halves per sample; parameters are
+ * nqm_mlp_param_count() floats: W1[16][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3. */
+typedef struct nqm_mlp nqm_mlp;
+uint32_t nqm_mlp_param_count(void);
+uint32_t nqm_mlp_feature_width(void);
+int nqm_mlp_create(nqm_ctx* ctx, uint64_t seed, nqm_mlp** out);
+void nqm_mlp_destroy(nqm_mlp* mlp);
+int nqm_mlp_set_params(nqm_mlp* mlp, const float* params);
+int nqm_mlp_get_params(nqm_mlp* mlp, float* params);
+/* Features of every prefix of a device pool into d_feats (device, count rows). */
+int nqm_mlp_features_pool(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uint8_t* d_pool,
+                          uint64_t count, void* d_feats);
+int nqm_mlp_predict(nqm_mlp* mlp, const void* d_feats, uint32_t count, float* d_out);
+/* Mean-squared-error gradient over `count` samples (host output, param order). */
+int nqm_mlp_gradient(nqm_mlp* mlp, const void* d_feats, const float* d_labels, uint32_t count,
+                     float* grad_out, float* loss_out);
+/* Minibatch Adam; a seeded split keeps the last val_frac of samples for
+ * validation. Losses are mean squared errors of log2(1 + nodes). */
+int nqm_mlp_train(nqm_mlp* mlp, const void* d_feats, const float* d_labels, uint32_t count,
+                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float val_frac,
+                  float* train_loss, float* val_loss);
+
 /* ---- batched persistent counting ----------------------------------------
  * Many independent solves (any mix of n and pre-placed prefixes) in one
  * persistent device pass. No reference counterpart: the reference counts one
@@ -114,6 +187,9 @@ typedef struct {
 #define NQM_TUNE_NO_MIRROR 0x1u  /* do not halve empty boards by mirror symmetry */
 #define NQM_TUNE_NO_DONATE 0x2u  /* no in-wave work donation (queue refill only) */
 #define NQM_TUNE_GRAPH 0x4u      /* run expand + drain as one captured HIP graph */
+#define NQM_TUNE_NO_ORDER 0x8u   /* with a model: split, but keep the queue order */
+
+struct nqm_mlp;
 
 typedef struct {
     uint32_t depth;        /* rows placed before the queue (per solve, capped
@@ -125,6 +201,13 @@ typedef struct {
                               queue relaunched (restart); 0 = drain in one launch */
     uint32_t flags;        /* NQM_TUNE_* */
     uint32_t stack_rows;   /* LDS stack rows per lane; 0 = what the deepest solve needs */
+    /* Learned splitting: with a model, after the expansion every item whose
+     * predicted log2(1 + nodes) exceeds split_log2 is split one row deeper,
+     * up to split_rounds times (0 = 2), then the queue is ordered largest
+     * predicted subtree first. split_log2 = 0 picks log2(sum / (8 x lanes)). */
+    struct nqm_mlp* model;
+    float split_log2;
+    uint32_t split_rounds;
 } nqm_tune;
 
 /* Count every solve. counts[i] = solutions extending solve i's prefix;
@@ -134,10 +217,49 @@ typedef struct {
 int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
                     const nqm_tune* tune, uint64_t* counts, uint64_t* nodes, nqm_stats* stats);
 
+/* ---- serving: continuous batching of count requests ---------------------
+ * No reference counterpart. Requests join a device-resident work queue
+ * between budgeted drain rounds; a request's result is ready the round its
+ * last queued item ends. Admission is by free counter slot and by queue
+ * memory. Each count equals that of the request solved alone. */
+typedef struct nqm_server nqm_server;
+typedef struct {
+    uint32_t max_active;   /* requests in flight (counter slots); 0 = 256 */
+    uint32_t depth;        /* rows a new request is expanded to; 0 = 6 */
+    uint64_t budget;       /* passes per wave per round; 0 = 16384 */
+    uint64_t queue_bytes;  /* device memory for queued work items; 0 = 1 GiB */
+    uint32_t flags;        /* NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE */
+    uint32_t reserved;
+} nqm_server_cfg;
+typedef struct {
+    uint64_t tag;          /* the caller's id of the request */
+    uint64_t count;        /* solutions extending the prefix */
+    uint64_t nodes;        /* placements below the prefix (full-tree numbers) */
+    uint32_t rounds;       /* drain rounds the request stayed in flight */
+    uint32_t reserved;
+    double admit_ms;       /* server clock at admission */
+    double done_ms;        /* server clock when the result was ready */
+} nqm_result;
+int nqm_server_create(nqm_ctx* ctx, const nqm_server_cfg* cfg, nqm_server** out);
+void nqm_server_destroy(nqm_server* srv);
+/* Offer requests in order; *admitted = how many (a prefix of reqs) were taken. */
+int nqm_server_submit(nqm_server* srv, const nqm_solve* reqs, const uint64_t* tags,
+                      uint32_t nreq, uint32_t* admitted);
+/* One drain round over everything queued; finished requests become pollable. */
+int nqm_server_step(nqm_server* srv, uint32_t* active_out);
+int nqm_server_poll(nqm_server* srv, nqm_result* out, uint32_t max, uint32_t* nout);
+int nqm_server_info(const nqm_server* srv, uint32_t* active, uint64_t* queued_items,
+                    uint64_t* rounds);
+/* Milliseconds since nqm_server_create (the clock of admit_ms / done_ms). */
+double nqm_server_clock_ms(const nqm_server* srv);
+
 /* ---- device memory helpers (so callers need no HIP headers) ------------ */
 int nqm_dev_alloc(nqm_ctx* ctx, size_t bytes, void** d_ptr);
 int nqm_dev_free(nqm_ctx* ctx, void* d_ptr);
 int nqm_memcpy_h2d(nqm_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0): the
+ * known-size reference for HBM byte counters. */
+int nqm_dev_copy(nqm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, double* kernel_ms);
 int nqm_memcpy_d2h(nqm_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
 
 #ifdef __cplusplus
