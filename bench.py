@@ -9,7 +9,11 @@ A step is one complete N=18 count (666,090,624 solutions) by one of two paths:
                 requested states: lock 8, 37,956,796 mirror-half prefixes),
                 generated on the device and resident in HBM before timing;
   persistent -- K2, expand + persistent queue drain with in-wave donation,
-                from the empty board (9 row-1 items) on the device.
+                from the empty board (9 row-1 items) on the device;
+  learned    -- K2 with the subtree-size MLP: items predicted large are split
+                deeper and the queue runs largest-predicted first. The model
+                is trained first (untimed) on the solver's own node counts of
+                every depth-4..6 prefix of N=13..16, by the device kernels.
 
 `value` is executed search nodes per second of the best path, over the timed
 steps (host wall clock, synchronized). Nodes are queen placements; the
@@ -19,7 +23,14 @@ time against the INT32 VALU peak (CUs x 64 lanes x engine clock) at 12
 integer ops per node. The cpu_baseline object times the CPU oracle (a port of
 the reference's counting loop) on a bounded sample of the same pool.
 
-  python bench.py [--steps K] [--warmup W] [--variant all|plain|persistent]
+  python bench.py [--steps K] [--warmup W] [--variant all|plain|persistent|learned]
+
+Other workloads (one JSON line each, not the headline metric):
+  --workload batch    512 seeded mixed solves (N 8..17, 0-4 pre-placed queens)
+                      in one persistent launch, plus the autotuner's sweep
+  --workload n19      N=19: plain static pool vs persistent vs learned
+  --workload serving  2000 seeded Poisson requests (N 10..20 ~ 1/N^2, 30% with
+                      a 1-3 queen prefix) by continuous batching at ~70% load
 """
 import argparse
 import json
@@ -44,10 +55,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--variant", default="all", choices=["all", "plain", "persistent"])
+    ap.add_argument("--variant", default="all", choices=["all", "plain", "persistent", "learned"])
+    ap.add_argument("--epochs", type=int, default=8, help="MLP training epochs (learned variant)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--requested-states", type=int, default=80_000_000)
+    ap.add_argument("--workload", default="n18", choices=["n18", "batch", "n19", "serving"])
+    ap.add_argument("--requests", type=int, default=2000, help="serving: stream length")
+    ap.add_argument("--load", type=float, default=0.7, help="serving: target utilisation")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter reference)")
     return ap.parse_args()
@@ -91,6 +106,142 @@ def run_persistent(ctx, nqm, args, tune):
     walls, kms, st = time_steps(step, args.steps, args.warmup)
     return {"path": "K2 persistent queue", "walls": walls, "kernel_ms": kms,
             "kernel_nodes": st.nodes, "blocks": st.blocks, "launches": st.launches}
+
+
+def run_learned(ctx, nqm, args):
+    from nqm import learned
+    t0 = time.perf_counter()
+    X, Y = learned.build_dataset(ctx)
+    model, info = learned.train(ctx, X, Y, epochs=args.epochs)
+    info["dataset_and_train_s"] = time.perf_counter() - t0
+    info["n18_depth6"] = learned.prediction_error(ctx, model, N, 6)
+    tune = nqm.make_tune(model=model)
+    try:
+        r = run_persistent(ctx, nqm, args, tune)
+    finally:
+        model.close()
+    r["path"] = "K2 persistent queue + learned splitting"
+    r["model"] = info
+    return r
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the dominant kernels, measured by rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs and calibrated on a copy kernel (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def workload_batch(ctx, nqm, args):
+    from nqm import autotune, workloads
+    solves = workloads.mixed_batch()
+    rows = {}
+    for name, kw in (("auto", {}), ("no_donate", {"flags": nqm.TUNE_NO_DONATE}),
+                     ("graph", {"flags": nqm.TUNE_GRAPH})):
+        tune = nqm.make_tune(**kw) if kw else None
+        ms = []
+        for _ in range(args.warmup + args.steps):
+            c, nd, st = ctx.solve_batch(solves, tune=tune)
+            ms.append(st.kernel_ms)
+        ms = ms[args.warmup:]
+        rows[name] = {"kernel_ms": sorted(ms)[len(ms) // 2], "total": int(c.sum()),
+                      "nodes_executed": st.nodes, "launches": st.launches}
+    best, trials = autotune.sweep(ctx, solves, depths=(3, 4, 5, 6, 7), waves=(4, 8, 12, 16),
+                                  repeats=2)
+    autotune.store(ctx, "mixed512", best, trials)
+    b = rows["auto"]
+    return {"metric": "512 mixed solves in one persistent launch: solves/s",
+            "value": 512 / (b["kernel_ms"] / 1e3), "unit": "solves/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": "512 solves, N~U{8..17}, k~U{0..4} seeded prefixes (splitmix64)"},
+            "variants": rows,
+            "autotune": {"best": {"depth": best.depth, "waves_per_cu": best.waves_per_cu,
+                                  "stack_rows": best.stack_rows},
+                         "trials": [{"depth": d, "waves_per_cu": w, "stack_rows": r, "ms": m}
+                                    for d, w, r, m in trials]}}
+
+
+def workload_n19(ctx, nqm, args):
+    from nqm import learned
+    n, expect = 19, 4968057848
+    out = {}
+    lock, _ = ctx.choose_lock(n, args.requested_states, nqm.MIRROR_WEIGHT)
+    buf, count = ctx.pool_generate_device(n, lock, nqm.MIRROR_WEIGHT)
+    try:
+        st = ctx.count_pool_device(n, lock, buf, count, flags=nqm.MIRROR_WEIGHT)
+        st = ctx.count_pool_device(n, lock, buf, count, flags=nqm.MIRROR_WEIGHT)
+        assert st.total == expect
+        out["plain"] = {"kernel_ms": st.kernel_ms, "lock": lock, "prefixes": count,
+                        "nodes_executed": st.nodes}
+    finally:
+        buf.free()
+    for name, tune in (("persistent", None),):
+        c, nd, st = ctx.solve_batch([(n, ())], tune=tune)
+        assert int(c[0]) == expect
+        c, nd, st = ctx.solve_batch([(n, ())], tune=tune)
+        out[name] = {"kernel_ms": st.kernel_ms, "nodes_executed": st.nodes, "nodes_full": int(nd[0])}
+    X, Y = learned.build_dataset(ctx)
+    model, info = learned.train(ctx, X, Y, epochs=args.epochs)
+    try:
+        tune = nqm.make_tune(model=model)
+        c, nd, st = ctx.solve_batch([(n, ())], tune=tune)
+        assert int(c[0]) == expect
+        c, nd, st = ctx.solve_batch([(n, ())], tune=tune)
+        out["learned"] = {"kernel_ms": st.kernel_ms, "nodes_executed": st.nodes,
+                          "model": info, "pred_error_n19_d6": learned.prediction_error(ctx, model, n, 6)}
+    finally:
+        model.close()
+    best = min(out, key=lambda k: out[k]["kernel_ms"])
+    return {"metric": "wall-s to exact count at N=19", "value": out[best]["kernel_ms"] / 1e3,
+            "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "int32",
+            "data": "synthetic", "config": {"workload": "N=19 empty board (4,968,057,848)",
+                                            "variant": best},
+            "variants": out,
+            "learned_vs_plain": out["plain"]["kernel_ms"] / out["learned"]["kernel_ms"],
+            "persistent_vs_plain": out["plain"]["kernel_ms"] / out["persistent"]["kernel_ms"]}
+
+
+def workload_serving(ctx, nqm, args):
+    from nqm import serving, workloads
+    known = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
+             17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884}  # test.cu:18
+    # capacity from a closed-loop run of the first 10% of the stream
+    probe = workloads.serving_stream(count=args.requests, rate=1.0)[: max(20, args.requests // 10)]
+    srv = ctx.server()
+    try:
+        res, wall = serving.replay(srv, [(0.0, n, p) for _, n, p in probe])
+    finally:
+        srv.close()
+    capacity = len(probe) / (wall / 1e3)
+    rate = args.load * capacity
+    reqs = workloads.serving_stream(count=args.requests, rate=rate)
+    srv = ctx.server()
+    try:
+        res, wall = serving.replay(srv, reqs)
+        rounds = srv.info()["rounds"]
+    finally:
+        srv.close()
+    summ = serving.latency_summary(res, wall)
+    bad = [t for t, (_, n, p) in enumerate(reqs) if not p and res[t][0] != known[n]]
+    # prefix requests: the same request alone through the batch path
+    alone = [t for t, (_, n, p) in enumerate(reqs) if p and n <= 16][:100]
+    for t in alone:
+        _, n, p = reqs[t]
+        c, _, _ = ctx.solve_batch([(n, tuple(p))])
+        if int(c[0]) != res[t][0]:
+            bad.append(t)
+    return {"metric": "serving: solves/s with p50/p99 latency", "value": summ["solves_per_s"],
+            "unit": "solves/s", "higher_is_better": True, "n_gpus": 1, "dtype": "int32",
+            "data": "synthetic",
+            "config": {"workload": f"{args.requests} Poisson requests, N 10..20 ~ 1/N^2, 30% with a "
+                                   f"1-3 queen prefix", "rate_per_s": rate,
+                       "capacity_probe_per_s": capacity, "target_load": args.load},
+            "latency": summ, "rounds": rounds,
+            "checked": {"empty_boards_vs_table": sum(1 for _, n, p in reqs if not p),
+                        "prefix_requests_vs_alone": len(alone), "mismatches": bad}}
 
 
 def cpu_baseline(seconds):
@@ -143,12 +294,22 @@ def main():
         return
     peak_ops = info["cus"] * 64 * info["clock_khz"] * 1e3  # INT32 lane-ops per second
 
+    if args.workload != "n18":
+        fn = {"batch": workload_batch, "n19": workload_n19, "serving": workload_serving}[args.workload]
+        line = fn(ctx, nqm, args)
+        line["device"] = info
+        ctx.close()
+        print(json.dumps(line), flush=True)
+        return
+
     lock, _ = ctx.choose_lock(N, args.requested_states, nqm.MIRROR_WEIGHT)
     results = {}
     if args.variant in ("all", "plain"):
         results["plain"] = run_plain(ctx, nqm, args, lock)
     if args.variant in ("all", "persistent"):
         results["persistent"] = run_persistent(ctx, nqm, args, None)
+    if args.variant in ("all", "learned"):
+        results["learned"] = run_learned(ctx, nqm, args)
 
     variants = {}
     for name, r in results.items():
@@ -160,6 +321,7 @@ def main():
             "kernel_nodes": r["kernel_nodes"], "blocks": r["blocks"],
             **({"lock": r["lock"], "prefixes": r["prefixes"]} if "lock" in r else {}),
             **({"launches": r["launches"]} if "launches" in r else {}),
+            **({"model": r["model"]} if "model" in r else {}),
         }
     best = min(variants, key=lambda k: variants[k]["ms_per_step"])
     b = variants[best]
@@ -182,14 +344,16 @@ def main():
                    "n": N, "variant": best, "path": b["path"]},
         "wall_s_to_count": b["wall_s_to_count"],
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_ops / 1e12,
-                     "unit": "TOP/s", "frac": achieved / peak_ops, "traffic": None,
+                     "unit": "TOP/s", "frac": achieved / peak_ops,
+                     "traffic": traffic_from_profiles().get(best),
                      "ops_per_node": OPS_PER_NODE, "kernel_nodes": b["kernel_nodes"],
                      "kernel_ms": b["kernel_ms"]},
         "variants": variants,
         "device": info,
     }
-    if "plain" in variants and "persistent" in variants:
-        line["persistent_vs_plain"] = variants["plain"]["ms_per_step"] / variants["persistent"]["ms_per_step"]
+    for v in ("persistent", "learned"):
+        if "plain" in variants and v in variants:
+            line[f"{v}_vs_plain"] = variants["plain"]["ms_per_step"] / variants[v]["ms_per_step"]
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -5,28 +5,32 @@
 // on a 64-lane ballot done-vote, counts are u64, one partial per block.
 //
 // MI355X design (not a translation of the CUDA kernel):
-//  * Relative u32 bitsets (cols, ld, rd) live in VGPRs; the candidate mask f
-//    of the current row is a fourth VGPR. A placement is
-//      b = f & -f; f ^= b; cols |= b; ld = (ld|b)<<1; rd = (rd|b)>>1;
-//      f' = ~(cols|ld|rd) & mask.
-//  * The backtrack stack is in LDS, one 16-byte entry {f_rest, cols, ld, rd}
-//    per open row, laid out [row][lane] so a wave's ds_write_b128 /
-//    ds_read_b128 touches 1 KiB of consecutive banks. An entry is pushed only
-//    when the row still has untried candidates, so a unit row (exactly one
-//    free cell) is placed and passed without touching LDS: that is the
-//    reference's DoubleSweep-light propagation (GPU_NQueens.cu:392-408), and
-//    it leaves the search tree, hence counts and node numbers, unchanged.
-//  * Every loop pass is one placement per live lane (pop first if the lane's
-//    row is exhausted). Stack entries always hold a non-empty f, so one pop
-//    suffices: no inner backtrack loop, no divergent while.
+//  * Relative u32 bitsets live in VGPRs: avail (free columns, the complement
+//    of the reference's queens_in_columns), ld, rd (columns attacked at the
+//    current row along the two diagonals) and f, the candidates of the
+//    current row not tried yet. The free set of a row is one v_bitop3:
+//    avail & ~ld & ~rd.
+//  * One pass = one placement per live lane: take b = f & -f and compute the
+//    child (avail^b, (ld|b)<<1, (rd|b)>>1) and its free set f'. A child with
+//    f' == 0 is a leaf: it is counted (a solution when avail^b == 0, i.e. all
+//    columns are taken) and the lane stays on its row. Only a child with a
+//    free cell is entered. A unit row (one candidate) is entered without
+//    saving anything: this is the reference's DoubleSweep-light propagation
+//    (GPU_NQueens.cu:392-408); the search tree, hence the counts and node
+//    numbers, are those of plain backtracking.
+//  * The backtrack stack is in LDS, one 16-byte entry {f, avail, ld, rd} per
+//    open row that still has untried candidates, laid out [row][lane] so a
+//    wave's ds_write_b128 / ds_read_b128 touches 1 KiB of consecutive banks.
+//    Leaves and unit rows never touch LDS; a lane pops only when its row is
+//    exhausted, and the popped row always has a candidate left.
 //  * Nodes are not counted per placement: a lane is live from pass 0 and
 //    places exactly one queen per pass, so its node count is the wave-uniform
 //    (scalar) pass counter at the moment it finishes.
-//  * Solutions are detected when a lane's candidate mask is empty: the board
-//    is full exactly when cols == mask (GPU_NQueens.cu:410-435).
 //  * Wave sum of u64 through cross-lane shuffles, LDS across the block, one
 //    store per block (the reference does 1024 same-address atomics,
 //    GPU_NQueens.cu:441-443).
+
+#include <type_traits>
 
 #include "nqm_internal.h"
 
@@ -61,6 +65,10 @@ __device__ __forceinline__ void block_sum2_u64(uint64_t& a, uint64_t& b, uint64_
     }
 }
 
+// kWide: u64 solution counter per lane. Without it the counter is u32, which
+// the host allows only when the whole board has fewer than 2^32 solutions
+// (n <= 18, test.cu:18), so no prefix can overflow it.
+template <bool kWide>
 __global__ void __launch_bounds__(kK1MaxThreads)
 k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, uint32_t depth,
                 uint32_t flags, uint64_t* __restrict__ block_sols,
@@ -71,11 +79,14 @@ k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, ui
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + tid;
     const uint32_t mask = (1u << n) - 1u;  // n <= 31
 
-    uint32_t cols = 0, ld = 0, rd = 0, f = 0;
+    // st = {f, avail, ld, rd}: the current row's untried candidates, the free
+    // columns and the two diagonal attack sets at that row
+    uint4 st = make_uint4(0, 0, 0, 0);
     uint32_t c0 = 0;
     bool live = false;
     if (gid < count) {
         const uint8_t* p = pool + gid * depth;
+        uint32_t cols = 0, ld = 0, rd = 0;
         bool ok = true;
         for (uint32_t r = 0; r < depth; ++r) {
             const uint32_t c = p[r];
@@ -90,45 +101,51 @@ k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, ui
             atomicOr(err, 1u);
         } else {
             live = true;
-            f = ~(cols | ld | rd) & mask;
+            const uint32_t avail = ~cols & mask;
+            st = make_uint4(avail & ~ld & ~rd, avail, ld, rd);
         }
     }
 
-    uint4* const my = lds_stack + tid;  // entry k of this lane: my[k * stride]
-    const uint32_t stride = blockDim.x;
-    uint32_t sp = 0;
+    // LDS byte address of this lane's next free stack slot; slots of one lane
+    // are blockDim.x * 16 bytes apart.
+    const uint32_t step = blockDim.x * (uint32_t)sizeof(uint4);
+    const uint32_t base = tid * (uint32_t)sizeof(uint4);
+    char* const lds = reinterpret_cast<char*>(lds_stack);
     uint64_t sols = 0, nodes = 0;
-    uint64_t pass = 0;  // wave-uniform
+    typename std::conditional<kWide, uint64_t, uint32_t>::type lsols = 0;
 
-    for (;;) {
-        if (live && f == 0) {
-            sols += (cols == mask);
-            if (sp == 0) {
-                live = false;
-                nodes = pass;
-            } else {
-                --sp;
-                const uint4 e = my[sp * stride];
-                f = e.x;
-                cols = e.y;
-                ld = e.z;
-                rd = e.w;
+    // The per-lane loop runs while the lane has work; the wave leaves it when
+    // its execution mask is empty, which is the 64-lane done-vote of the
+    // reference (GPU_NQueens.cu:359-360).
+    if (live) {
+        // a complete prefix (depth == n) is one solution with nothing below it
+        lsols = st.y == 0 ? 1u : 0u;
+        uint32_t top = base;
+        for (;;) {
+            if (st.x == 0) {  // row exhausted: resume the nearest open row
+                if (top == base) break;
+                top -= step;
+                st = *reinterpret_cast<const uint4*>(lds + top);
             }
-        }
-        if (__ballot(live) == 0) break;  // 64-lane done-vote
-        if (live) {
-            const uint32_t b = f & (0u - f);
-            f ^= b;
-            if (f) {
-                my[sp * stride] = make_uint4(f, cols, ld, rd);
-                ++sp;
+            const uint32_t b = st.x & (0u - st.x);
+            st.x ^= b;
+            const uint32_t a2 = st.y ^ b;
+            const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
+            const uint32_t r2 = (st.w | b) >> 1;
+            const uint32_t f2 = a2 & ~l2 & ~r2;
+            lsols += (a2 == 0);  // all columns taken: a solution (a leaf)
+            const bool go = f2 != 0;
+            if (go && st.x != 0) {  // entering the child: save this row, it has more
+                *reinterpret_cast<uint4*>(lds + top) = st;
+                top += step;
             }
-            cols |= b;
-            ld = (ld | b) << 1;
-            rd = (rd | b) >> 1;
-            f = ~(cols | ld | rd) & mask;
+            st.x = go ? f2 : st.x;
+            st.y = go ? a2 : st.y;
+            st.z = go ? l2 : st.z;
+            st.w = go ? r2 : st.w;
+            ++nodes;  // one placement per pass
         }
-        ++pass;
+        sols = lsols;
     }
 
     if (gid < count) {
@@ -165,9 +182,14 @@ hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_
                      uint64_t count, uint32_t n, uint32_t depth, uint32_t flags,
                      uint64_t* d_block_sols, uint64_t* d_block_nodes, uint64_t* d_per_sols,
                      uint64_t* d_per_nodes, uint32_t* d_err) {
-    hipLaunchKernelGGL(k1_count_kernel, dim3(plan.blocks), dim3(plan.threads), plan.lds_bytes,
-                       stream, d_pool, count, n, depth, flags, d_block_sols, d_block_nodes,
-                       d_per_sols, d_per_nodes, d_err);
+    if (n <= 18)
+        hipLaunchKernelGGL(k1_count_kernel<false>, dim3(plan.blocks), dim3(plan.threads), plan.lds_bytes,
+                           stream, d_pool, count, n, depth, flags, d_block_sols, d_block_nodes,
+                           d_per_sols, d_per_nodes, d_err);
+    else
+        hipLaunchKernelGGL(k1_count_kernel<true>, dim3(plan.blocks), dim3(plan.threads), plan.lds_bytes,
+                           stream, d_pool, count, n, depth, flags, d_block_sols, d_block_nodes,
+                           d_per_sols, d_per_nodes, d_err);
     return hipGetLastError();
 }
 

@@ -29,6 +29,7 @@
 //    the spill queue (restart), which also lets new work join between rounds.
 
 #include <algorithm>
+#include <type_traits>
 
 #include "nqm_internal.h"
 #include "nqm_items.h"
@@ -164,85 +165,108 @@ __device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, ui
     nodes = 0;
 }
 
+// kWide: u64 solution counter per lane, else u32 (allowed by the host only
+// when every solve's board has fewer than 2^32 solutions, n <= 18).
+template <bool kWide>
 __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
+    typedef typename std::conditional<kWide, uint64_t, uint32_t>::type sol_t;
     extern __shared__ uint4 lds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    const uint32_t stride = blockDim.x;
-    uint4* const stack = lds;                                   // [stack_rows][stride]
-    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds + (size_t)a.stack_rows * stride) +
+    char* const stack = reinterpret_cast<char*>(lds);           // [stack_rows][blockDim] x 16 B
+    const uint32_t step = blockDim.x * (uint32_t)sizeof(uint4);  // next slot of the same lane
+    const uint32_t base = tid * (uint32_t)sizeof(uint4);
+    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds + (size_t)a.stack_rows * blockDim.x) +
                                wave * 64u;                      // 64 u32 per wave
     const uint32_t count = *a.item_count;
 
-    // lane state
-    uint32_t f = 0, cols = 0, ld = 0, rd = 0, mask = 0, solve = 0, ws = 0;
-    uint32_t sbase = 0, stop = 0;  // stack slots [sbase, stop)
+    // lane state: st = {f, avail, ld, rd} of the current row; the lane's open
+    // rows are the LDS slots [bot, top)
+    uint4 st = make_uint4(0, 0, 0, 0);
+    uint32_t mask = 0, solve = 0, ws = 0;
+    uint32_t bot = base, top = base;
     bool live = false;
-    uint64_t start = 0, isols = 0;
+    uint64_t pnodes = 0;  // placements of the current piece
+    sol_t psols = 0;      // solutions of the current piece
     uint32_t acc_solve = 0;
-    uint64_t acc_sols = 0, acc_nodes = 0, exec = 0;
+    uint64_t acc_sols = 0, acc_nodes = 0, exec_nodes = 0;
 
     bool queue_open = true;
-    uint64_t pass = 0, next_refill = 0;
+    uint64_t pass = 0;  // wave-uniform: passes run so far
 
     for (;;) {
-        // (1) exhausted row: count a full board, then pop or finish the piece
-        if (live && f == 0) {
-            isols += (cols == mask);
-            if (stop == sbase) {
+        // ---- pass budget: spill every open row as an item, then leave
+        if (a.budget && pass >= a.budget) {
+            if (live) {
+                // open rows: the current one if it has candidates, then the stack
+                const uint32_t cur = st.x != 0 ? 1u : 0u;
+                const uint32_t nitems = cur + (top - bot) / step;
+                const uint32_t o = nitems ? atomicAdd(a.spill_count, nitems) : 0u;
+                if (nitems && (uint64_t)o + nitems > a.spill_cap) {
+                    atomicOr(a.err, 4u);
+                } else if (nitems) {
+                    const uint32_t n = 31u - __clz(mask + 1u);
+                    Item it;
+                    it.solve = solve;
+                    it.pad0 = it.pad1 = 0;
+                    uint4 e = st;
+                    uint32_t sa = bot;
+                    for (uint32_t j = 0; j < nitems; ++j) {
+                        if (j || !cur) {
+                            e = *reinterpret_cast<const uint4*>(stack + sa);
+                            sa += step;
+                        }
+                        it.f = e.x;
+                        it.cols = ~e.y & mask;
+                        it.ld = e.z & mask;
+                        it.rd = e.w;
+                        it.meta = item_meta(n, (uint32_t)__popc(it.cols), ws);
+                        a.spill[o + j] = it;
+                    }
+                }
+                exec_nodes += pnodes;
+                acc_nodes += pnodes << ws;
+                acc_sols += (uint64_t)psols << ws;
                 live = false;
-                const uint64_t nd = pass - start;
-                exec += nd;
-                acc_nodes += nd << ws;
-                acc_sols += isols << ws;
-                isols = 0;
-            } else {
-                --stop;
-                const uint4 e = stack[stop * stride + tid];
-                f = e.x;
-                cols = e.y;
-                ld = e.z;
-                rd = e.w;
             }
+            break;
         }
 
-        // (2) refill idle lanes: global queue first, then donation in-wave
+        // ---- refill idle lanes: the global queue first, then in-wave donation
         uint64_t idle = __ballot(!live);
-        const uint32_t nidle = __popcll(idle);
-        if (nidle && (nidle == 64u || (nidle >= a.refill_idle && pass >= next_refill))) {
+        if (idle) {
+            const uint32_t nidle = __popcll(idle);
             if (queue_open) {
                 uint32_t qbase = 0;
                 if (lane == 0) qbase = atomicAdd(a.head, nidle);
                 qbase = __shfl(qbase, 0, 64);
-                const uint32_t avail = qbase >= count ? 0u : min(nidle, count - qbase);
-                if (avail < nidle) queue_open = false;
+                const uint32_t got = qbase >= count ? 0u : min(nidle, count - qbase);
+                if (got < nidle) queue_open = false;
                 const uint32_t r = lane_rank(idle);
-                if (!live && r < avail) {
+                if (!live && r < got) {
                     const Item it = a.items[qbase + r];
                     if (it.solve != acc_solve) {
                         flush_acc(a, acc_solve, acc_sols, acc_nodes);
                         acc_solve = it.solve;
                     }
-                    f = it.f;
-                    cols = it.cols;
-                    ld = it.ld;
-                    rd = it.rd;
-                    solve = it.solve;
                     mask = (1u << meta_n(it.meta)) - 1u;
+                    const uint32_t avail = ~it.cols & mask;
+                    st = make_uint4(it.f, avail, it.ld, it.rd);
+                    solve = it.solve;
                     ws = meta_wshift(it.meta);
-                    sbase = stop = 0;
-                    start = pass;
-                    isols = 0;
+                    bot = top = base;
+                    pnodes = 0;
+                    psols = 0;
                     live = true;
                 }
                 idle = __ballot(!live);
             }
             if (idle && a.donate) {
-                const uint64_t donors = __ballot(live && stop > sbase);
+                const bool is_donor = live && top != bot;
+                const uint64_t donors = __ballot(is_donor);
                 const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
                 if (k) {
-                    const bool is_donor = live && stop > sbase;
                     const uint32_t dr = lane_rank(donors);
                     if (is_donor && dr < k) wscratch[dr] = lane;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -251,99 +275,112 @@ __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
                     const uint32_t ir = lane_rank(idle);
                     const bool recv = !live && ir < k;
                     const uint32_t dl = recv ? wscratch[ir] : lane;
-                    const uint32_t d_base = __shfl(sbase, dl, 64);
+                    const uint32_t d_bot = __shfl(bot, dl, 64);
                     const uint32_t d_mask = __shfl(mask, dl, 64);
                     const uint32_t d_solve = __shfl(solve, dl, 64);
                     const uint32_t d_ws = __shfl(ws, dl, 64);
                     if (recv) {
-                        const uint4 e = stack[d_base * stride + (wave * 64u + dl)];
+                        // the donor's shallowest open row, straight from its LDS slot
+                        st = *reinterpret_cast<const uint4*>(stack + d_bot);
                         if (d_solve != acc_solve) {
                             flush_acc(a, acc_solve, acc_sols, acc_nodes);
                             acc_solve = d_solve;
                         }
-                        f = e.x;
-                        cols = e.y;
-                        ld = e.z;
-                        rd = e.w;
                         mask = d_mask;
                         solve = d_solve;
                         ws = d_ws;
-                        sbase = stop = 0;
-                        start = pass;
-                        isols = 0;
+                        bot = top = base;
+                        pnodes = 0;
+                        psols = 0;
                         live = true;
                     }
-                    if (is_donor && dr < k) ++sbase;
+                    if (is_donor && dr < k) bot += step;
                     // keep the scratch stable until every receiver has read it
                     __builtin_amdgcn_wave_barrier();
                 }
             }
-            const uint32_t still = __popcll(__ballot(!live));
-            next_refill = (still >= a.refill_idle && still < 64u) ? pass + 64u : pass;
         }
+        const uint64_t live_mask = __ballot(live);
+        if (live_mask == 0 && !queue_open) break;
 
-        if (__ballot(live) == 0) {
-            if (!queue_open) break;
-            continue;  // an all-idle wave refills on the next pass
+        // ---- search: live lanes place one queen per pass (the K1 step) until
+        // their piece ends, or until enough lanes idle to make a refill worth
+        // it (only while a refill can find work), or the pass budget is hit
+        const uint32_t nlive = __popcll(live_mask);
+        const bool can_refill = queue_open || (a.donate && __ballot(live && top != bot) != 0);
+        uint32_t floor_live = 0;
+        uint64_t limit = a.budget ? a.budget - pass : 0xffffffffffffull;
+        if (can_refill) {
+            if (nlive > 64u - a.refill_idle)
+                floor_live = 64u - a.refill_idle;  // leave once refill_idle lanes idle
+            else
+                limit = limit < 256u ? limit : 256u;  // idle lanes already: retry soon
         }
-
-        // (3) pass budget: spill every open row as an item, then leave
-        if (a.budget && pass >= a.budget) {
-            if (live) {
-                const uint32_t nitems = 1u + (stop - sbase);
-                const uint32_t o = atomicAdd(a.spill_count, nitems);
-                if ((uint64_t)o + nitems > a.spill_cap) {
-                    atomicOr(a.err, 4u);
-                } else {
-                    const uint32_t meta_base = item_meta(31u - __clz(mask + 1u), 0u, ws);
-                    Item it;
-                    it.solve = solve;
-                    it.pad0 = it.pad1 = 0;
-                    it.f = f;
-                    it.cols = cols;
-                    it.ld = ld;
-                    it.rd = rd;
-                    it.meta = meta_base | ((uint32_t)__popc(cols) << 8);
-                    a.spill[o] = it;
-                    for (uint32_t s = sbase; s < stop; ++s) {
-                        const uint4 e = stack[s * stride + tid];
-                        it.f = e.x;
-                        it.cols = e.y;
-                        it.ld = e.z;
-                        it.rd = e.w;
-                        it.meta = meta_base | ((uint32_t)__popc(e.y) << 8);
-                        a.spill[o + 1u + (s - sbase)] = it;
+        // thresholds are wave-uniform: keep them in scalar registers
+        floor_live = __builtin_amdgcn_readfirstlane(floor_live);
+        const uint32_t lim32 = __builtin_amdgcn_readfirstlane(
+            (uint32_t)(limit < 0x7fffffffull ? limit : 0x7fffffffull));
+        const uint64_t pn0 = pnodes;
+        if (live) {
+            uint32_t ran = 0;  // passes of this phase (uniform among running lanes)
+            for (;;) {
+                bool fin = false;
+                if (st.x == 0) {  // row exhausted: resume the nearest open row
+                    if (top == bot) {
+                        fin = true;  // the piece is done
+                    } else {
+                        top -= step;
+                        st = *reinterpret_cast<const uint4*>(stack + top);
                     }
                 }
-                const uint64_t nd = pass - start;
-                exec += nd;
-                acc_nodes += nd << ws;
-                acc_sols += isols << ws;
-                isols = 0;
-                live = false;
+                if (!fin) {
+                    const uint32_t b = st.x & (0u - st.x);
+                    st.x ^= b;
+                    const uint32_t a2 = st.y ^ b;
+                    const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
+                    const uint32_t r2 = (st.w | b) >> 1;
+                    const uint32_t f2 = a2 & ~l2 & ~r2;
+                    psols += (a2 == 0);
+                    const bool go = f2 != 0;
+                    if (go && st.x != 0) {
+                        *reinterpret_cast<uint4*>(stack + top) = st;
+                        top += step;
+                    }
+                    st.x = go ? f2 : st.x;
+                    st.y = go ? a2 : st.y;
+                    st.z = go ? l2 : st.z;
+                    st.w = go ? r2 : st.w;
+                    ++pnodes;
+                }
+                ++ran;
+                // one exit: finished lanes, or everyone when enough lanes are
+                // done or the phase limit is reached
+                const uint32_t running = __popcll(__ballot(!fin));
+                if (fin || running <= floor_live || ran >= lim32) {
+                    live = !fin;
+                    break;
+                }
             }
-            break;
-        }
-
-        // (4) one placement per live lane (the K1 step)
-        if (live) {
-            const uint32_t b = f & (0u - f);
-            f ^= b;
-            if (f) {
-                stack[stop * stride + tid] = make_uint4(f, cols, ld, rd);
-                ++stop;
+            if (!live) {  // account the finished piece
+                exec_nodes += pnodes;
+                acc_nodes += pnodes << ws;
+                acc_sols += (uint64_t)psols << ws;
             }
-            cols |= b;
-            ld = (ld | b) << 1;
-            rd = (rd | b) >> 1;
-            f = ~(cols | ld | rd) & mask;
         }
-        ++pass;
+        // the phase length: the most placements any lane made in it
+        uint64_t rmax = pnodes - pn0;
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t o = __shfl_xor(rmax, off, 64);
+            rmax = o > rmax ? o : rmax;
+        }
+        rmax = rmax ? rmax : 1;  // a phase that only closed pieces still counts
+        pass += rmax;
     }
 
     flush_acc(a, acc_solve, acc_sols, acc_nodes);
-    exec = wave_sum64(exec);
-    if (lane == 0 && exec) atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec);
+    exec_nodes = wave_sum64(exec_nodes);
+    if (lane == 0 && exec_nodes)
+        atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec_nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -489,14 +526,18 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.donate = p.donate;
     a.pad = 0;
     a.budget = p.budget;
-    hipLaunchKernelGGL(k2_drain_kernel, dim3(blocks), dim3(threads),
-                       k2_lds_bytes(p.stack_rows, threads), s, a);
+    if (p.wide)
+        hipLaunchKernelGGL(k2_drain_kernel<true>, dim3(blocks), dim3(threads),
+                           k2_lds_bytes(p.stack_rows, threads), s, a);
+    else
+        hipLaunchKernelGGL(k2_drain_kernel<false>, dim3(blocks), dim3(threads),
+                           k2_lds_bytes(p.stack_rows, threads), s, a);
     return hipGetLastError();
 }
 
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel, (int)threads, lds_bytes) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<true>, (int)threads, lds_bytes) !=
         hipSuccess) {
         (void)hipGetLastError();
         return NQM_EDEVICE;

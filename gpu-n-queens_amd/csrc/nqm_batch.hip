@@ -139,7 +139,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     const double t0 = now_ms();
     nqm_tune t{};
     if (tune_in) t = *tune_in;
-    if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH))
+    if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH | NQM_TUNE_NO_ORDER))
         return fail(ctx, NQM_EINVAL, "unknown tune flags 0x%x", t.flags);
     const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 16u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
@@ -334,6 +334,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     p.refill_idle = refill;
     p.donate = (t.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
     p.budget = t.budget;
+    p.wide = max_n >= 19;
     p.head = d_head;
     p.spill_count = d_spill_count;
     p.item_count = d_qcount;
@@ -455,4 +456,65 @@ extern "C" int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t n
     if (!solves || nsolves == 0 || !counts) return nqm::fail(ctx, NQM_EINVAL, "empty batch");
     NQM_HIP(ctx, hipSetDevice(ctx->device));
     return nqm::solve_batch_impl(ctx, solves, nsolves, tune, counts, nodes, stats);
+}
+
+extern "C" int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
+                            const nqm_tune_space* space, nqm_tune* best, nqm_tune_result* results,
+                            uint32_t max_results, uint32_t* nresults) {
+    if (!ctx || !solves || !nsolves || !space || !best) return NQM_EINVAL;
+    NQM_HIP(ctx, hipSetDevice(ctx->device));
+    auto list = [](const uint32_t* v) {
+        std::vector<uint32_t> out;
+        for (int i = 0; i < 8 && v[i]; ++i) out.push_back(v[i]);
+        if (out.empty()) out.push_back(0);  // automatic
+        return out;
+    };
+    const std::vector<uint32_t> depths = list(space->depths);
+    const std::vector<uint32_t> waves = list(space->waves_per_cu);
+    const std::vector<uint32_t> rows = list(space->stack_rows);
+    const uint32_t reps = space->repeats ? space->repeats : 3u;
+    std::vector<uint64_t> ref(nsolves), got(nsolves);
+    bool have_ref = false, have_best = false;
+    double best_ms = 0;
+    uint32_t nres = 0;
+    for (uint32_t d : depths)
+        for (uint32_t w : waves)
+            for (uint32_t r : rows) {
+                nqm_tune t{};
+                t.depth = d;
+                t.waves_per_cu = w;
+                t.stack_rows = r;
+                t.flags = space->base_flags & ~(uint32_t)NQM_TUNE_GRAPH;
+                std::vector<double> ms;
+                for (uint32_t k = 0; k < reps; ++k) {
+                    nqm_stats st{};
+                    int rc = nqm::solve_batch_impl(ctx, solves, nsolves, &t, got.data(), nullptr, &st);
+                    if (rc != NQM_OK) return rc;
+                    if (!have_ref) {
+                        ref = got;
+                        have_ref = true;
+                    } else if (got != ref) {
+                        return nqm::fail(ctx, NQM_ERANGE, "tune depth=%u waves=%u rows=%u changed a count", d,
+                                         w, r);
+                    }
+                    ms.push_back(st.kernel_ms);
+                }
+                std::sort(ms.begin(), ms.end());
+                const double med = ms[ms.size() / 2];
+                if (results && nres < max_results) {
+                    results[nres].depth = d;
+                    results[nres].waves_per_cu = w;
+                    results[nres].stack_rows = r;
+                    results[nres].pad = 0;
+                    results[nres].ms = med;
+                }
+                ++nres;
+                if (!have_best || med < best_ms) {
+                    best_ms = med;
+                    *best = t;
+                    have_best = true;
+                }
+            }
+    if (nresults) *nresults = std::min(nres, max_results);
+    return NQM_OK;
 }

@@ -87,6 +87,7 @@ struct K2DrainParams {
     uint32_t refill_idle = 16;
     uint32_t donate = 1;
     uint64_t budget = 0;
+    bool wide = true;  // u64 per-lane solution counters (needed when some n >= 19)
 };
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);

@@ -343,6 +343,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         p.refill_idle = 16;
         p.donate = (s->cfg.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
         p.budget = spill_ok ? budget : 0;
+        p.wide = true;
         NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
         NQM_HIP(ctx, k2_append_leftover(st, p.items, qslot, p.head, p.spill, p.spill_count, p.spill_cap, p.err));
         // items still queued per slot

@@ -95,6 +95,18 @@ def make_tune(**kw):
     return t
 
 
+class TuneSpace(ctypes.Structure):
+    """nqm_tune_space: candidate lists (zero-terminated, up to 8 each)."""
+    _fields_ = [("depths", ctypes.c_uint32 * 8), ("waves_per_cu", ctypes.c_uint32 * 8),
+                ("stack_rows", ctypes.c_uint32 * 8), ("repeats", ctypes.c_uint32),
+                ("base_flags", ctypes.c_uint32)]
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
+                ("stack_rows", ctypes.c_uint32), ("pad", ctypes.c_uint32), ("ms", ctypes.c_double)]
+
+
 class ServerCfg(ctypes.Structure):
     """nqm_server_cfg (zero = default for every field)."""
     _fields_ = [("max_active", ctypes.c_uint32), ("depth", ctypes.c_uint32),
@@ -163,6 +175,9 @@ _SIGNATURES = {
     "nqm_mlp_train": ([VP, VP, VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
                        ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
                        ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+    "nqm_autotune": ([VP, ctypes.POINTER(Solve), ctypes.c_uint32, ctypes.POINTER(TuneSpace),
+                      ctypes.POINTER(Tune), ctypes.POINTER(TuneResult), ctypes.c_uint32,
+                      ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "nqm_server_create": ([VP, ctypes.POINTER(ServerCfg), ctypes.POINTER(VP)], ctypes.c_int),
     "nqm_server_destroy": ([VP], None),
     "nqm_server_submit": ([VP, ctypes.POINTER(Solve), U64P, ctypes.c_uint32,

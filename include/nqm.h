@@ -217,6 +217,28 @@ typedef struct {
 int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
                     const nqm_tune* tune, uint64_t* counts, uint64_t* nodes, nqm_stats* stats);
 
+/* Autotuner: runs the batch under every combination of the candidate
+ * expansion depths, waves per CU and LDS stack-row caps (zero-terminated
+ * lists; an empty list means "automatic only"), `repeats` times each, and
+ * returns the fastest tune (median kernel time). Every combination must give
+ * the same counts, or the call fails with NQM_ERANGE. results (optional,
+ * capacity max_results) receive one row per combination:
+ * depth, waves_per_cu, stack_rows, median ms. */
+typedef struct {
+    uint32_t depths[8];
+    uint32_t waves_per_cu[8];
+    uint32_t stack_rows[8];
+    uint32_t repeats;      /* 0 = 3 */
+    uint32_t base_flags;   /* NQM_TUNE_* applied to every trial */
+} nqm_tune_space;
+typedef struct {
+    uint32_t depth, waves_per_cu, stack_rows, pad;
+    double ms;
+} nqm_tune_result;
+int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
+                 const nqm_tune_space* space, nqm_tune* best, nqm_tune_result* results,
+                 uint32_t max_results, uint32_t* nresults);
+
 /* ---- serving: continuous batching of count requests ---------------------
  * No reference counterpart. Requests join a device-resident work queue
  * between budgeted drain rounds; a request's result is ready the round its

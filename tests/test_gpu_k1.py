@@ -108,7 +108,7 @@ def test_unpruned_and_full_prefixes(ctx):
     sols = []
     for p in oracle.pool_generate(n, 7, mirror=False):
         free = [c for c in range(n) if c not in p]
-        q = list(p) + free
+        q = [int(c) for c in p] + free
         if all(abs(q[i] - q[j]) != j - i for i in range(n) for j in range(i + 1, n)):
             sols.append(q)
     full = np.array(sols, np.uint8)
