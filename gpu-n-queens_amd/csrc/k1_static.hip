@@ -166,8 +166,8 @@ k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, ui
 int k1_plan(const nqm_ctx* ctx, uint32_t n, uint32_t depth, uint64_t count, K1Launch* plan) {
     // Entries are pushed at rows depth..n-2 only (row n-1 has <= 1 candidate).
     const uint32_t rows = (n > depth + 1) ? (n - 1 - depth) : 1;
-    uint32_t threads = kK1MaxThreads;
-    while (threads > 64 && (size_t)rows * threads * sizeof(uint4) > ctx->lds_per_block) threads >>= 1;
+    // one-wave workgroups pack the per-lane LDS stacks of a CU most tightly
+    uint32_t threads = 64;
     const size_t lds = (size_t)rows * threads * sizeof(uint4);
     if (lds > ctx->lds_per_block) return NQM_EUNSUPPORTED;
     const uint64_t blocks = (count + threads - 1) / threads;

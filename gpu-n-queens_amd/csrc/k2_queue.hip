@@ -57,6 +57,24 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
     return v;
 }
 
+// Add one u64 per lane into arr[solve] (lanes with add == 0 contribute
+// nothing). When every contributing lane of the wave has the same solve --
+// the common case, queues are grouped by solve -- the wave sums first and
+// issues one atomic instead of 64 to the same address. All lanes must call.
+__device__ __forceinline__ void add_per_solve(uint64_t* __restrict__ arr, uint32_t solve, uint64_t add,
+                                              uint32_t lane) {
+    const uint64_t act = __ballot(add != 0);
+    if (act == 0) return;
+    const uint32_t first = (uint32_t)__ffsll((long long)act) - 1u;
+    const uint32_t s0 = __shfl(solve, first, 64);
+    if (__ballot(add != 0 && solve != s0) == 0) {
+        const uint64_t tot = wave_sum64(add);
+        if (lane == first) atomicAdd((unsigned long long*)&arr[s0], (unsigned long long)tot);
+    } else if (add) {
+        atomicAdd((unsigned long long*)&arr[solve], (unsigned long long)add);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // expand: one level of children for every item below its solve's target row
 // ---------------------------------------------------------------------------
@@ -96,10 +114,7 @@ k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_co
             }
         }
         // nodes placed by this level, weighted to full-tree numbers
-        if (expand) {
-            atomicAdd((unsigned long long*)&solve_nodes[it.solve],
-                      (unsigned long long)placed << meta_wshift(it.meta));
-        }
+        add_per_solve(solve_nodes, it.solve, expand ? (uint64_t)placed << meta_wshift(it.meta) : 0u, lane);
         const uint64_t wplaced = wave_sum64(placed);
         if (lane == 0 && wplaced) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wplaced);
         const uint32_t incl = wave_incl_scan(nch, lane);
@@ -377,7 +392,8 @@ __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
         pass += rmax;
     }
 
-    flush_acc(a, acc_solve, acc_sols, acc_nodes);
+    add_per_solve(a.solve_sols, acc_solve, acc_sols, lane);
+    add_per_solve(a.solve_nodes, acc_solve, acc_nodes, lane);
     exec_nodes = wave_sum64(exec_nodes);
     if (lane == 0 && exec_nodes)
         atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec_nodes);

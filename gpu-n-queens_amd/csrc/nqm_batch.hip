@@ -114,7 +114,8 @@ struct Batch {
 // requested cap allow.
 int drain_shape(nqm_ctx* ctx, uint32_t stack_rows, const nqm_tune& t, uint32_t* blocks,
                 uint32_t* threads) {
-    uint32_t th = t.threads ? t.threads : 256u;
+    // one-wave workgroups pack the LDS stacks of a CU most tightly
+    uint32_t th = t.threads ? t.threads : 64u;
     if (th != 64 && th != 128 && th != 256) return fail(ctx, NQM_EINVAL, "threads=%u", th);
     while (th > 64 && k2_lds_bytes(stack_rows, th) > ctx->lds_per_block) th >>= 1;
     if (k2_lds_bytes(stack_rows, th) > ctx->lds_per_block)

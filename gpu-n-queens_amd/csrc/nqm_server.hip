@@ -48,7 +48,8 @@ struct nqm_server {
     uint32_t slots_n = 0;
     uint64_t cap_items = 0;   // admission limit of queued items
     uint64_t buf_items = 0;   // physical capacity of each queue buffer
-    nqm::DevBuf q[2], stage[2], counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list;
+    nqm::DevBuf q[2], stage[2], counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list,
+        lvl_nodes;
     int cur = 0;
     uint32_t qcount = 0;
     struct Slot {
@@ -67,7 +68,7 @@ struct nqm_server {
     uint64_t rounds_total = 0;
     ~nqm_server() {
         for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &counts, &head, &spill_count, &err, &target,
-                               &sols, &nodes, &exec, &outst, &slot_list})
+                               &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes})
             if (d->ptr) (void)hipFree(d->ptr);
     }
     double clock_ms() const {
@@ -79,9 +80,12 @@ namespace nqm {
 
 namespace {
 
-// Expand one request's initial items (host) into the staging buffer, up to
-// its target row. Returns the item count left in stage[*which].
-int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint32_t* items_out, int* which) {
+// Expand one request's initial items (host) into the staging buffer, one row
+// per level up to its target row, but never past `room` items: a level that
+// would overflow is dropped and the request stays at the shallower row.
+// Returns the item count left in stage[*which].
+int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t room, uint32_t* items_out,
+                  int* which) {
     nqm_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     int rc;
@@ -128,9 +132,6 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint32_t* it
     }
     const uint32_t depth = s->cfg.depth ? s->cfg.depth : 6u;
     const uint32_t target = std::min(n - 1, std::max(row, depth));
-    sl.row = row < target && !init.empty() ? target : row;
-    uint8_t tgt[1] = {(uint8_t)target};
-    NQM_HIP(ctx, hipMemcpyAsync(static_cast<uint8_t*>(s->target.ptr) + slot, tgt, 1, hipMemcpyHostToDevice, st));
     uint32_t count = (uint32_t)init.size();
     if ((rc = ensure_dev(ctx, s->stage[0], std::max<size_t>(count, 1) * sizeof(Item))) != NQM_OK) return rc;
     if (count)
@@ -138,20 +139,39 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint32_t* it
     uint32_t* cnt = static_cast<uint32_t*>(s->counts.ptr);
     NQM_HIP(ctx, k2_fill_u32(st, cnt, 1, count));
     int w = 0;
+    uint32_t* in_slot = cnt;
     for (uint32_t r = row; r < target && count; ++r) {
         const uint64_t cap = (uint64_t)count * n;
-        if (cap > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "request expansion exceeds 2^31 items");
+        if (cap > 0x7fffffffull) break;
+        // expand exactly one row: target of this slot = r + 1
+        uint8_t tgt[1] = {(uint8_t)(r + 1)};
+        NQM_HIP(ctx, hipMemcpyAsync(static_cast<uint8_t*>(s->target.ptr) + slot, tgt, 1, hipMemcpyHostToDevice, st));
         if ((rc = ensure_dev(ctx, s->stage[w ^ 1], cap * sizeof(Item))) != NQM_OK) return rc;
-        NQM_HIP(ctx, k2_fill_u32(st, cnt + 1 + (w ^ 1), 1, 0));
+        uint32_t* out_slot = cnt + 1 + (w ^ 1);
+        NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
+        // the level's placements go to scratch counters until the level is kept:
+        // lvl_nodes[slot] (weighted, per solve) and lvl_nodes[slots_n] (executed)
+        uint64_t* const lvl = static_cast<uint64_t*>(s->lvl_nodes.ptr);
+        NQM_HIP(ctx, k2_fill_u64(st, lvl + slot, 1));
+        NQM_HIP(ctx, k2_fill_u64(st, lvl + s->slots_n, 1));
         const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-        NQM_HIP(ctx, k2_expand(st, grid, static_cast<Item*>(s->stage[w].ptr), w == 0 && r == row ? cnt : cnt + 1 + w,
-                               static_cast<Item*>(s->stage[w ^ 1].ptr), cnt + 1 + (w ^ 1), (uint32_t)cap,
-                               static_cast<uint8_t*>(s->target.ptr), static_cast<uint64_t*>(s->nodes.ptr),
-                               static_cast<uint64_t*>(s->exec.ptr), static_cast<uint32_t*>(s->err.ptr)));
-        NQM_HIP(ctx, hipMemcpyAsync(&count, cnt + 1 + (w ^ 1), 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, k2_expand(st, grid, static_cast<Item*>(s->stage[w].ptr), in_slot,
+                               static_cast<Item*>(s->stage[w ^ 1].ptr), out_slot, (uint32_t)cap,
+                               static_cast<uint8_t*>(s->target.ptr), lvl, lvl + s->slots_n,
+                               static_cast<uint32_t*>(s->err.ptr)));
+        uint32_t next = 0;
+        uint64_t placed = 0;
+        NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&placed, lvl + slot, 8, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
+        if (next > room) break;  // keep the shallower level
+        sl.host_nodes += placed;
+        count = next;
+        in_slot = out_slot;
+        row = r + 1;
         w ^= 1;
     }
+    sl.row = row;
     *items_out = count;
     *which = w;
     return NQM_OK;
@@ -199,6 +219,7 @@ int nqm_server_create(nqm_ctx* ctx, const nqm_server_cfg* cfg, nqm_server** out)
         if (rc == NQM_OK) rc = ensure_dev(ctx, *d, s->buf_items * sizeof(Item));
     for (DevBuf* d : {&s->counts, &s->head, &s->spill_count, &s->err, &s->exec})
         if (rc == NQM_OK) rc = ensure_dev(ctx, *d, 64);
+    if (rc == NQM_OK) rc = ensure_dev(ctx, s->lvl_nodes, ((size_t)s->slots_n + 1) * 8);
     if (rc == NQM_OK) rc = ensure_dev(ctx, s->target, s->slots_n);
     for (DevBuf* d : {&s->sols, &s->nodes})
         if (rc == NQM_OK) rc = ensure_dev(ctx, *d, (size_t)s->slots_n * 8);
@@ -266,9 +287,12 @@ int nqm_server_submit(nqm_server* s, const nqm_solve* reqs, const uint64_t* tags
         NQM_HIP(ctx, hipGetLastError());
         uint32_t items = 0;
         int which = 0;
-        int rc = stage_request(s, q, slot, &items, &which);
+        // room left in the queue; an empty queue always takes a request (its
+        // unexpanded items are at most ceil(n/2) <= 16)
+        const uint64_t room = s->cap_items > s->qcount ? s->cap_items - s->qcount : 0;
+        int rc = stage_request(s, q, slot, room, &items, &which);
         if (rc != NQM_OK) return rc;
-        if ((uint64_t)s->qcount + items > s->cap_items) break;  // no queue memory: try later
+        if ((uint64_t)s->qcount + items > s->cap_items && s->qcount > 0) break;  // no room: try later
         if (items)
             NQM_HIP(ctx, hipMemcpyAsync(static_cast<Item*>(s->q[s->cur].ptr) + s->qcount, s->stage[which].ptr,
                                         (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice, st));
@@ -310,8 +334,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         uint32_t blocks = 0, threads = 0;
         {
             // same grid rule as the batch path
-            uint32_t th = 256;
-            while (th > 64 && k2_lds_bytes(rows, th) > ctx->lds_per_block) th >>= 1;
+            uint32_t th = 64;  // one-wave workgroups pack LDS stacks most tightly
             int per_cu = 0;
             if ((rc = k2_max_blocks_per_cu(th, k2_lds_bytes(rows, th), &per_cu)) != NQM_OK)
                 return fail(ctx, rc, "occupancy query failed");

@@ -73,16 +73,23 @@ __global__ void __launch_bounds__(kGenThreads)
 gen_count_kernel(const GenItem* __restrict__ in, uint32_t count, uint32_t mask,
                  uint32_t* __restrict__ block_sums, unsigned long long* __restrict__ per_col) {
     __shared__ uint32_t sh[8];
+    __shared__ unsigned long long cols[32];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (per_col && threadIdx.x < 32) cols[threadIdx.x] = 0;
+    __syncthreads();
     uint32_t c = 0;
     if (i < count) {
         const GenItem it = in[i];
         c = __popc(child_mask(it, mask));
-        if (per_col && c) atomicAdd(&per_col[it.row0], (unsigned long long)c);
+        // per row-0 column counts: LDS first, one global atomic per column
+        // per block (items are grouped by column, so few columns per block)
+        if (per_col && c) atomicAdd(&cols[it.row0], (unsigned long long)c);
     }
     uint32_t tot;
     (void)block_excl_scan(c, sh, &tot);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+    if (per_col && threadIdx.x < 32 && cols[threadIdx.x])
+        atomicAdd(&per_col[threadIdx.x], cols[threadIdx.x]);
 }
 
 // One workgroup: exclusive scan of nblocks sums in place; writes the total.

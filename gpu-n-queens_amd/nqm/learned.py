@@ -72,12 +72,15 @@ def prediction_error(ctx, model, n, depth):
     y = np.log2(pn.astype(np.float64) + 1.0)
     err = pred.astype(np.float64) - y
     r2 = 1.0 - float(np.mean(err ** 2)) / float(np.var(y))
+    ry = np.argsort(np.argsort(y))
+    rp = np.argsort(np.argsort(pred))
+    spearman = float(np.corrcoef(ry, rp)[0, 1])
     # rank agreement of the largest subtrees (what LPT ordering needs)
     k = max(1, cnt // 100)
     top_true = set(np.argsort(-y)[:k].tolist())
     top_pred = set(np.argsort(-pred)[:k].tolist())
     return {"n": n, "depth": depth, "prefixes": int(cnt), "mae_log2": float(np.mean(np.abs(err))),
-            "rmse_log2": float(np.sqrt(np.mean(err ** 2))), "r2": r2,
+            "rmse_log2": float(np.sqrt(np.mean(err ** 2))), "r2": r2, "spearman": spearman,
             "top1pct_overlap": len(top_true & top_pred) / k}
 
 

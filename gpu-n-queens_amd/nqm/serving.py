@@ -15,6 +15,7 @@ def replay(srv, requests, on_result=None, time_scale=1.0):
     out = {}
     t_start = srv.clock_ms()
     waiting = []  # arrived, not yet admitted
+    stalled = 0
     while i < len(pending) or waiting or srv.info()["active"]:
         now = srv.clock_ms() - t_start
         while i < len(pending) and pending[i][1][0] * 1e3 * time_scale <= now:
@@ -24,6 +25,10 @@ def replay(srv, requests, on_result=None, time_scale=1.0):
             took = srv.submit([(n, tuple(p)) for _, (_, n, p) in waiting], [tag for tag, _ in waiting])
             waiting = waiting[took:]
         info = srv.info()
+        # an idle server that refuses a waiting request would spin forever
+        stalled = stalled + 1 if (waiting and info["active"] == 0 and info["queued_items"] == 0) else 0
+        if stalled > 3:
+            raise RuntimeError("server admits nothing while idle")
         if info["active"] == 0 and not waiting:
             if i < len(pending):  # idle: sleep until the next arrival
                 nxt = pending[i][1][0] * 1e3 * time_scale - (srv.clock_ms() - t_start)

@@ -45,8 +45,9 @@ def test_learned_with_restart_budget(ctx, small_model, known_totals):
 def test_model_generalises_to_a_larger_board(ctx, small_model):
     # N=14 at depth 5 leaves 9 rows, a depth the training boards also had
     # (N=12 depth 3, N=13 depth 4); the board itself was never seen.
+    # what largest-first ordering needs is the ranking of subtree sizes
     e = learned.prediction_error(ctx, small_model, 14, 5)
-    assert e["r2"] > 0.3, e
+    assert e["spearman"] > 0.6, e
     assert e["top1pct_overlap"] > 0.2, e
 
 

@@ -48,14 +48,14 @@ def test_mix_equals_alone(ctx):
 
 
 def test_overlapping_requests_share_rounds(ctx, known_totals):
-    # a long request and many short ones that arrive while it runs
-    reqs = [(0.0, 16, ())] + [(0.001 * i, 8 + i % 4, ()) for i in range(1, 40)]
-    srv = ctx.server(budget=4000)
+    # a long request (many short rounds) and short ones that arrive while it runs
+    reqs = [(0.0, 17, ())] + [(0.0002 * i, 8 + i % 4, ()) for i in range(1, 40)]
+    srv = ctx.server(budget=200)
     try:
         res, _ = serving.replay(srv, reqs)
     finally:
         srv.close()
-    assert res[0][0] == known_totals[16]
+    assert res[0][0] == known_totals[17]
     for tag in range(1, 40):
         assert res[tag][0] == known_totals[8 + tag % 4]
     # short requests finished while the long one was still in flight
