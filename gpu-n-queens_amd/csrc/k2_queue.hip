@@ -36,7 +36,6 @@
 
 namespace nqm {
 
-constexpr uint32_t kK2Threads = 256;
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -78,77 +77,138 @@ __device__ __forceinline__ void add_per_solve(uint64_t* __restrict__ arr, uint32
 // ---------------------------------------------------------------------------
 // expand: one level of children for every item below its solve's target row
 // ---------------------------------------------------------------------------
+// Block b owns the contiguous chunk [b*c, (b+1)*c) of the input, c = ceil(cnt
+// / gridDim). Pass 1 counts the chunk's children, one atomic reserves the
+// chunk's output range, pass 2 writes the children in input order. Node
+// counts are folded per thread and flushed once. Same-address atomics are
+// therefore per block, not per wave or item.
+
+// Which input items expand, and into how many live children.
+struct ExpandRule {
+    const uint8_t* target_row;
+    const float* pred;
+    float thresh;
+    __device__ __forceinline__ bool expands(const Item& it, uint32_t i) const {
+        const uint32_t n = meta_n(it.meta), row = meta_row(it.meta);
+        // by prediction (split the large subtrees, never the last rows) or
+        // up to the solve's target row
+        return pred ? (pred[i] > thresh && row + 2 < n) : (row < target_row[it.solve]);
+    }
+};
+
+__device__ __forceinline__ uint32_t live_children(const Item& it) {
+    const uint32_t mask = (1u << meta_n(it.meta)) - 1u;
+    uint32_t f = it.f, nch = 0;
+    while (f) {
+        const uint32_t b = f & (0u - f);
+        f ^= b;
+        nch += (~((it.cols | b) | ((it.ld | b) << 1) | ((it.rd | b) >> 1)) & mask) != 0u;
+    }
+    return nch;
+}
+
+// Block-wide exclusive scan of one u32 per thread (blockDim = 256); *total
+// receives the block sum. Ends with the scratch free for reuse.
+__device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t* scratch, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(x, lane);
+    if (lane == 63u) scratch[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4u; ++w) {
+        const uint32_t v = scratch[w];
+        before += w < wave ? v : 0u;
+        sum += v;
+    }
+    __syncthreads();
+    *total = sum;
+    return before + incl - x;
+}
+
 __global__ void __launch_bounds__(256)
 k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
                  Item* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
                  const uint8_t* __restrict__ target_row, uint64_t* __restrict__ solve_nodes,
                  uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err,
-                 const float* __restrict__ pred, float thresh) {
+                 const float* __restrict__ pred, const float* __restrict__ thresh_p) {
+    __shared__ uint32_t scratch[4];
+    __shared__ uint32_t s_base;
     const uint32_t cnt = *in_count;
+    const ExpandRule rule{target_row, pred, pred ? *thresh_p : 0.f};
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t step = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < cnt; base += step) {
-        const uint32_t i = base + threadIdx.x;
+    const uint32_t chunk = (cnt + gridDim.x - 1u) / gridDim.x;
+    const uint32_t c0 = min(cnt, blockIdx.x * chunk), c1 = min(cnt, c0 + chunk);
+
+    // pass 1: children of the chunk
+    uint32_t mine = 0;
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += 256u) {
+        const Item it = in[i];
+        mine += rule.expands(it, i) ? live_children(it) : 1u;
+    }
+    uint32_t block_total = 0;
+    (void)block_scan256(mine, scratch, &block_total);
+    if (threadIdx.x == 0) s_base = block_total ? atomicAdd(out_count, block_total) : 0u;
+    __syncthreads();
+    const uint32_t obase = s_base;
+    const bool fits = (uint64_t)obase + block_total <= out_cap;
+    if (!fits && threadIdx.x == 0) atomicOr(err, 2u);
+
+    // pass 2: write in input order; count the placements of expanded items
+    uint32_t acc_solve = 0;
+    uint64_t acc_nodes = 0, acc_exec = 0;
+    uint32_t o = obase;
+    for (uint32_t b0 = c0; b0 < c1; b0 += 256u) {  // block-uniform trip count
+        const uint32_t i = b0 + threadIdx.x;
         Item it{};
-        uint32_t nch = 0, n = 0, row = 0, mask = 0, placed = 0;
         bool expand = false;
-        if (i < cnt) {
+        uint32_t nch = 0;
+        if (i < c1) {
             it = in[i];
-            n = meta_n(it.meta);
-            row = meta_row(it.meta);
-            mask = (1u << n) - 1u;
-            // by prediction (split the large subtrees, never the last rows)
-            // or up to the solve's target row
-            expand = pred ? (pred[i] > thresh && row + 2 < n) : (row < target_row[it.solve]);
-            if (expand) {
-                uint32_t f = it.f;
-                placed = __popc(f);
-                while (f) {
-                    const uint32_t b = f & (0u - f);
-                    f ^= b;
-                    const uint32_t cf = ~((it.cols | b) | ((it.ld | b) << 1) | ((it.rd | b) >> 1)) & mask;
-                    nch += (cf != 0u);
-                }
-            } else {
-                nch = 1;
-            }
+            expand = rule.expands(it, i);
+            nch = expand ? live_children(it) : 1u;
         }
-        // nodes placed by this level, weighted to full-tree numbers
-        add_per_solve(solve_nodes, it.solve, expand ? (uint64_t)placed << meta_wshift(it.meta) : 0u, lane);
-        const uint64_t wplaced = wave_sum64(placed);
-        if (lane == 0 && wplaced) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wplaced);
-        const uint32_t incl = wave_incl_scan(nch, lane);
-        const uint32_t total = __shfl(incl, 63, 64);
-        uint32_t obase = 0;
-        if (lane == 0 && total) obase = atomicAdd(out_count, total);
-        obase = __shfl(obase, 0, 64);
-        uint32_t o = obase + incl - nch;
-        if (nch && (uint64_t)obase + total > out_cap) {
-            atomicOr(err, 2u);
+        uint32_t step_total = 0;
+        uint32_t w = o + block_scan256(nch, scratch, &step_total);
+        o += step_total;
+        if (i >= c1) continue;
+        if (expand) {
+            const uint32_t placed = (uint32_t)__popc(it.f);
+            const uint32_t ws = meta_wshift(it.meta);
+            if (it.solve != acc_solve && acc_nodes) {
+                atomicAdd((unsigned long long*)&solve_nodes[acc_solve], (unsigned long long)acc_nodes);
+                acc_nodes = 0;
+            }
+            acc_solve = it.solve;
+            acc_nodes += (uint64_t)placed << ws;
+            acc_exec += placed;
+        }
+        if (!fits) continue;
+        if (!expand) {
+            out[w] = it;
             continue;
         }
-        if (i < cnt) {
-            if (!expand) {
-                out[o] = it;
-            } else {
-                uint32_t f = it.f;
-                const uint32_t ws = meta_wshift(it.meta);
-                while (f) {
-                    const uint32_t b = f & (0u - f);
-                    f ^= b;
-                    Item c;
-                    c.cols = it.cols | b;
-                    c.ld = ((it.ld | b) << 1) & mask;
-                    c.rd = (it.rd | b) >> 1;
-                    c.f = ~(c.cols | c.ld | c.rd) & mask;
-                    c.solve = it.solve;
-                    c.meta = item_meta(n, row + 1, ws);
-                    c.pad0 = c.pad1 = 0;
-                    if (c.f) out[o++] = c;
-                }
-            }
+        const uint32_t n = meta_n(it.meta), row = meta_row(it.meta);
+        const uint32_t mask = (1u << n) - 1u;
+        const uint32_t ws = meta_wshift(it.meta);
+        uint32_t f = it.f;
+        while (f) {
+            const uint32_t b = f & (0u - f);
+            f ^= b;
+            Item c;
+            c.cols = it.cols | b;
+            c.ld = ((it.ld | b) << 1) & mask;
+            c.rd = (it.rd | b) >> 1;
+            c.f = ~(c.cols | c.ld | c.rd) & mask;
+            c.solve = it.solve;
+            c.meta = item_meta(n, row + 1, ws);
+            c.pad0 = c.pad1 = 0;
+            if (c.f) out[w++] = c;
         }
     }
+    add_per_solve(solve_nodes, acc_solve, acc_nodes, lane);
+    const uint64_t wexec = wave_sum64(acc_exec);
+    if (lane == 0 && wexec) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wexec);
 }
 
 // ---------------------------------------------------------------------------
@@ -180,43 +240,145 @@ __device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, ui
     nodes = 0;
 }
 
-// kWide: u64 solution counter per lane, else u32 (allowed by the host only
-// when every solve's board has fewer than 2^32 solutions, n <= 18).
-template <bool kWide>
-__global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
-    typedef typename std::conditional<kWide, uint64_t, uint32_t>::type sol_t;
+// LDS of one drain workgroup of T lanes (T = 64, 128 or 256):
+//   slot k, word j of lane t at 4 (t + 1) + k * 16T + j * 4T bytes: word j of
+//                          one slot is a plane of T consecutive dwords, and the
+//                          slot stride 16T is a multiple of the 64 banks, so a
+//                          wave's access touches 64 consecutive banks whatever
+//                          the stack depth of each lane (no bank conflicts);
+//   the inert entry        at 0, 4T, 8T, 12T: address 0 is below every lane,
+//                          the other three are words 0..2 of slot 0 of lane
+//                          T-1, which is reserved and never holds work;
+//   after rows slots:      64 u32 per wave of donation scratch.
+// The inert entry {f=0, avail=1, ld=~0, rd=2} is a row that is never done:
+// placing its (empty) candidate gives avail 1 (no solution), no child cell
+// (so nothing is entered or counted) and leaves the entry as it was. A lane
+// whose stack runs empty pops it (the pop saturates at address 0) and from
+// then on runs harmless passes until the next refill; an idle lane is simply
+// one whose stack pointer is 0. The search loop therefore needs no per-lane
+// exit and no per-pass vote.
+__device__ __forceinline__ uint4 inert_entry() { return make_uint4(0u, 1u, 0xffffffffu, 2u); }
+
+template <uint32_t T>
+struct Stack {
+    static constexpr uint32_t kPlane = 4u * T;      // bytes per word plane
+    static constexpr uint32_t kSlot = 4u * kPlane;  // bytes per slot (all lanes)
+    // byte address of slot 0 of lane t (word 0)
+    static __device__ __forceinline__ uint32_t base(uint32_t t) { return 4u * (t + 1u); }
+    static size_t bytes(uint32_t rows) { return (size_t)rows * kSlot + (size_t)T * sizeof(uint32_t); }
+};
+
+// Stack slots are addressed by their LDS byte offset; the dynamic LDS of the
+// drain is the only LDS of the kernel, so offset == address. The words of a
+// slot are {avail, ld, rd, f} in planes 0..3.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+template <uint32_t T>
+__device__ __forceinline__ uint4 slot_load(uint32_t addr) {
+    const lds_u32* p = reinterpret_cast<const lds_u32*>((size_t)addr);
+    constexpr uint32_t w = Stack<T>::kPlane / 4u;
+    return make_uint4(p[3 * w], p[0], p[w], p[2 * w]);
+}
+template <uint32_t T>
+__device__ __forceinline__ void slot_store(uint32_t addr, uint4 st) {
+    lds_u32* p = reinterpret_cast<lds_u32*>((size_t)addr);
+    constexpr uint32_t w = Stack<T>::kPlane / 4u;
+    p[0] = st.y;
+    p[w] = st.z;
+    p[2 * w] = st.w;
+    p[3 * w] = st.x;
+}
+
+constexpr uint32_t kPassUnroll = 4;          // passes between two votes
+constexpr uint32_t kPhaseCap = 1u << 26;     // passes per phase: u32 counters cannot wrap
+constexpr uint32_t kGrabMax = 256;           // queue items one wave takes per atomic
+
+// One search phase: every lane places one queen per pass (idle lanes spin on
+// the inert entry) until at most floor_live lanes hold a piece, or lim32
+// passes ran (checked every kPassUnroll passes). kWaveSols: psols is the
+// wave's solution count (same in every lane), else the lane's own.
+template <uint32_t T, bool kWaveSols>
+__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t floor_live, uint32_t lim32,
+                                             uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
+    constexpr uint32_t step = Stack<T>::kSlot;
+    for (;;) {
+#pragma unroll
+        for (uint32_t u = 0; u < kPassUnroll; ++u) {
+            if (st.x == 0) {  // row exhausted: resume the nearest open row
+                top = __builtin_elementwise_sub_sat(top, step);
+                st = slot_load<T>(top);
+            }
+            const uint32_t b = st.x & (0u - st.x);
+            st.x ^= b;
+            const uint32_t a2 = st.y ^ b;
+            const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
+            const uint32_t r2 = (st.w | b) >> 1;
+            const uint32_t f2 = a2 & ~l2 & ~r2;
+            if (kWaveSols)
+                psols += (uint32_t)__popcll(__ballot(a2 == 0));
+            else
+                psols += (a2 == 0);
+            // entering a row places each of its candidates once: count them
+            // all now (f2 == 0 when the child is not entered)
+            pnodes += (uint32_t)__popc(f2);
+            const bool go = f2 != 0;
+            if (go && st.x != 0) {
+                slot_store<T>(top, st);
+                top += step;
+            }
+            st.x = go ? f2 : st.x;
+            st.y = go ? a2 : st.y;
+            st.z = go ? l2 : st.z;
+            st.w = go ? r2 : st.w;
+        }
+        ran += kPassUnroll;
+        const uint32_t running = __popcll(__ballot(top != 0));
+        if (running <= floor_live || ran >= lim32) break;
+    }
+}
+
+template <uint32_t T>
+__global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     extern __shared__ uint4 lds[];
+    constexpr uint32_t step = Stack<T>::kSlot;  // next slot of the same lane
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    char* const stack = reinterpret_cast<char*>(lds);           // [stack_rows][blockDim] x 16 B
-    const uint32_t step = blockDim.x * (uint32_t)sizeof(uint4);  // next slot of the same lane
-    const uint32_t base = tid * (uint32_t)sizeof(uint4);
-    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds + (size_t)a.stack_rows * blockDim.x) +
-                               wave * 64u;                      // 64 u32 per wave
+    const uint32_t base = Stack<T>::base(tid);  // > 0: 0 means "no piece"
+    const bool reserved = tid == T - 1u;        // its slot 0 holds the inert entry
+    uint32_t* const wscratch =
+        reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (size_t)a.stack_rows * step) + wave * 64u;
+    // the stack offsets below are LDS addresses: the dynamic LDS must start at 0
+    if ((uint32_t)(size_t)(lds_u32*)lds != 0u) {
+        if (tid == 0) atomicOr(a.err, 16u);
+        return;
+    }
+    if (tid == 0) slot_store<T>(0u, inert_entry());
+    __syncthreads();
     const uint32_t count = *a.item_count;
+    const uint32_t waves = gridDim.x * (T / 64u);
 
-    // lane state: st = {f, avail, ld, rd} of the current row; the lane's open
-    // rows are the LDS slots [bot, top)
-    uint4 st = make_uint4(0, 0, 0, 0);
+    // lane state: st = {f, avail, ld, rd} of the current row; the open rows
+    // are the LDS slots [bot, top); slots in [base, bot) were donated and now
+    // hold the inert entry. top == 0: the lane holds no piece.
+    uint4 st = inert_entry();
     uint32_t mask = 0, solve = 0, ws = 0;
-    uint32_t bot = base, top = base;
-    bool live = false;
-    uint64_t pnodes = 0;  // placements of the current piece
-    sol_t psols = 0;      // solutions of the current piece
+    uint32_t bot = 0, top = 0;
     uint32_t acc_solve = 0;
     uint64_t acc_sols = 0, acc_nodes = 0, exec_nodes = 0;
 
+    // wave-uniform queue state: items [lq, lq_end) are taken from the queue
+    // and not yet started; rem is what the queue held after the last grab
     bool queue_open = true;
+    uint32_t lq = 0, lq_end = 0, rem = count;
     uint64_t pass = 0;  // wave-uniform: passes run so far
 
     for (;;) {
-        // ---- pass budget: spill every open row as an item, then leave
+        // ---- pass budget: spill every open row and every held item, then leave
         if (a.budget && pass >= a.budget) {
-            if (live) {
+            if (top != 0) {
                 // open rows: the current one if it has candidates, then the stack
                 const uint32_t cur = st.x != 0 ? 1u : 0u;
-                const uint32_t nitems = cur + (top - bot) / step;
+                const uint32_t nitems = cur + (top > bot ? (top - bot) / step : 0u);
                 const uint32_t o = nitems ? atomicAdd(a.spill_count, nitems) : 0u;
                 if (nitems && (uint64_t)o + nitems > a.spill_cap) {
                     atomicOr(a.err, 4u);
@@ -227,9 +389,10 @@ __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
                     it.pad0 = it.pad1 = 0;
                     uint4 e = st;
                     uint32_t sa = bot;
+                    uint32_t counted = 0;
                     for (uint32_t j = 0; j < nitems; ++j) {
                         if (j || !cur) {
-                            e = *reinterpret_cast<const uint4*>(stack + sa);
+                            e = slot_load<T>(sa);
                             sa += step;
                         }
                         it.f = e.x;
@@ -238,65 +401,104 @@ __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
                         it.rd = e.w;
                         it.meta = item_meta(n, (uint32_t)__popc(it.cols), ws);
                         a.spill[o + j] = it;
+                        counted += (uint32_t)__popc(e.x);
                     }
+                    // these candidates were counted when their row was entered;
+                    // the round that drains the spilled items counts them again
+                    exec_nodes -= counted;
+                    acc_nodes -= (uint64_t)counted << ws;
                 }
-                exec_nodes += pnodes;
-                acc_nodes += pnodes << ws;
-                acc_sols += (uint64_t)psols << ws;
-                live = false;
+                top = 0;
+            }
+            if (lq < lq_end) {  // taken but never started: back to the queue
+                const uint32_t nloc = lq_end - lq;
+                uint32_t o = 0;
+                if (lane == 0) o = atomicAdd(a.spill_count, nloc);
+                o = __shfl(o, 0, 64);
+                if ((uint64_t)o + nloc > a.spill_cap) {
+                    if (lane == 0) atomicOr(a.err, 4u);
+                } else {
+                    for (uint32_t j = lane; j < nloc; j += 64u) a.spill[o + j] = a.items[lq + j];
+                }
+                lq = lq_end;
             }
             break;
         }
 
-        // ---- refill idle lanes: the global queue first, then in-wave donation
-        uint64_t idle = __ballot(!live);
+        // ---- refill idle lanes: held items, then the global queue (one
+        // atomic per grab of up to kGrabMax items), then in-wave donation
+        uint64_t idle = __ballot(top == 0 && !reserved);
         if (idle) {
             const uint32_t nidle = __popcll(idle);
-            if (queue_open) {
-                uint32_t qbase = 0;
-                if (lane == 0) qbase = atomicAdd(a.head, nidle);
-                qbase = __shfl(qbase, 0, 64);
-                const uint32_t got = qbase >= count ? 0u : min(nidle, count - qbase);
-                if (got < nidle) queue_open = false;
+            if (queue_open || lq < lq_end) {
+                const uint32_t take_local = min(nidle, lq_end - lq);
+                uint32_t g = 0, got = 0;
+                if (take_local < nidle && queue_open) {
+                    // guided self-scheduling: a share of what is left, at
+                    // least what the idle lanes need
+                    const uint32_t need = nidle - take_local;
+                    uint32_t want = rem / (2u * waves);
+                    want = want < need ? need : (want > kGrabMax ? kGrabMax : want);
+                    if (lane == 0) g = atomicAdd(a.head, want);
+                    g = __shfl(g, 0, 64);
+                    got = g >= count ? 0u : min(want, count - g);
+                    if (got < want) queue_open = false;
+                    rem = g >= count ? 0u : count - g - got;
+                }
                 const uint32_t r = lane_rank(idle);
-                if (!live && r < got) {
-                    const Item it = a.items[qbase + r];
+                uint32_t idx = 0xffffffffu;
+                if (top == 0 && !reserved) {
+                    if (r < take_local)
+                        idx = lq + r;
+                    else if (r - take_local < got)
+                        idx = g + (r - take_local);
+                }
+                if (idx != 0xffffffffu) {
+                    const Item it = a.items[idx];
                     if (it.solve != acc_solve) {
                         flush_acc(a, acc_solve, acc_sols, acc_nodes);
                         acc_solve = it.solve;
                     }
                     mask = (1u << meta_n(it.meta)) - 1u;
-                    const uint32_t avail = ~it.cols & mask;
-                    st = make_uint4(it.f, avail, it.ld, it.rd);
+                    st = make_uint4(it.f, ~it.cols & mask, it.ld, it.rd);
                     solve = it.solve;
                     ws = meta_wshift(it.meta);
                     bot = top = base;
-                    pnodes = 0;
-                    psols = 0;
-                    live = true;
+                    // the item's own candidates are placements of this piece
+                    const uint32_t pn = (uint32_t)__popc(it.f);
+                    exec_nodes += pn;
+                    acc_nodes += (uint64_t)pn << ws;
                 }
-                idle = __ballot(!live);
+                lq += take_local;
+                if (got) {  // the held range was used up: keep the rest of the grab
+                    lq = g + min(got, nidle - take_local);
+                    lq_end = g + got;
+                }
+                idle = __ballot(top == 0 && !reserved);
             }
             if (idle && a.donate) {
-                const bool is_donor = live && top != bot;
+                const bool is_donor = top > bot;
                 const uint64_t donors = __ballot(is_donor);
                 const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
                 if (k) {
                     const uint32_t dr = lane_rank(donors);
-                    if (is_donor && dr < k) wscratch[dr] = lane;
+                    const bool gives = is_donor && dr < k;
+                    if (gives) wscratch[dr] = lane;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const uint32_t ir = lane_rank(idle);
-                    const bool recv = !live && ir < k;
+                    const bool recv = top == 0 && !reserved && ir < k;
                     const uint32_t dl = recv ? wscratch[ir] : lane;
                     const uint32_t d_bot = __shfl(bot, dl, 64);
                     const uint32_t d_mask = __shfl(mask, dl, 64);
                     const uint32_t d_solve = __shfl(solve, dl, 64);
                     const uint32_t d_ws = __shfl(ws, dl, 64);
                     if (recv) {
-                        // the donor's shallowest open row, straight from its LDS slot
-                        st = *reinterpret_cast<const uint4*>(stack + d_bot);
+                        // the donor's shallowest open row, straight from its LDS
+                        // slot; its candidates were counted when the donor
+                        // entered that row
+                        st = slot_load<T>(d_bot);
                         if (d_solve != acc_solve) {
                             flush_acc(a, acc_solve, acc_sols, acc_nodes);
                             acc_solve = d_solve;
@@ -305,91 +507,57 @@ __global__ void __launch_bounds__(kK2Threads) k2_drain_kernel(DrainArgs a) {
                         solve = d_solve;
                         ws = d_ws;
                         bot = top = base;
-                        pnodes = 0;
-                        psols = 0;
-                        live = true;
                     }
-                    if (is_donor && dr < k) bot += step;
-                    // keep the scratch stable until every receiver has read it
+                    // every receiver has read its entry before the donors overwrite it
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (gives) {
+                        slot_store<T>(bot, inert_entry());
+                        bot += step;
+                    }
                     __builtin_amdgcn_wave_barrier();
                 }
             }
         }
-        const uint64_t live_mask = __ballot(live);
-        if (live_mask == 0 && !queue_open) break;
+        const uint64_t live_mask = __ballot(top != 0);
+        const bool have_items = queue_open || lq < lq_end;
+        if (live_mask == 0 && !have_items) break;
 
-        // ---- search: live lanes place one queen per pass (the K1 step) until
-        // their piece ends, or until enough lanes idle to make a refill worth
-        // it (only while a refill can find work), or the pass budget is hit
+        // ---- search: every lane places one queen per pass (the K1 step;
+        // idle lanes spin on the inert entry) until enough lanes idle to make
+        // a refill worth it (only while a refill can find work), every lane
+        // is idle, or the phase limit is reached
         const uint32_t nlive = __popcll(live_mask);
-        const bool can_refill = queue_open || (a.donate && __ballot(live && top != bot) != 0);
+        const bool can_refill = have_items || (a.donate && __ballot(top > bot) != 0);
         uint32_t floor_live = 0;
-        uint64_t limit = a.budget ? a.budget - pass : 0xffffffffffffull;
+        uint64_t limit = a.budget ? a.budget - pass : (uint64_t)kPhaseCap;
         if (can_refill) {
             if (nlive > 64u - a.refill_idle)
                 floor_live = 64u - a.refill_idle;  // leave once refill_idle lanes idle
             else
                 limit = limit < 256u ? limit : 256u;  // idle lanes already: retry soon
         }
-        // thresholds are wave-uniform: keep them in scalar registers
         floor_live = __builtin_amdgcn_readfirstlane(floor_live);
         const uint32_t lim32 = __builtin_amdgcn_readfirstlane(
-            (uint32_t)(limit < 0x7fffffffull ? limit : 0x7fffffffull));
-        const uint64_t pn0 = pnodes;
-        if (live) {
-            uint32_t ran = 0;  // passes of this phase (uniform among running lanes)
-            for (;;) {
-                bool fin = false;
-                if (st.x == 0) {  // row exhausted: resume the nearest open row
-                    if (top == bot) {
-                        fin = true;  // the piece is done
-                    } else {
-                        top -= step;
-                        st = *reinterpret_cast<const uint4*>(stack + top);
-                    }
-                }
-                if (!fin) {
-                    const uint32_t b = st.x & (0u - st.x);
-                    st.x ^= b;
-                    const uint32_t a2 = st.y ^ b;
-                    const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
-                    const uint32_t r2 = (st.w | b) >> 1;
-                    const uint32_t f2 = a2 & ~l2 & ~r2;
-                    psols += (a2 == 0);
-                    const bool go = f2 != 0;
-                    if (go && st.x != 0) {
-                        *reinterpret_cast<uint4*>(stack + top) = st;
-                        top += step;
-                    }
-                    st.x = go ? f2 : st.x;
-                    st.y = go ? a2 : st.y;
-                    st.z = go ? l2 : st.z;
-                    st.w = go ? r2 : st.w;
-                    ++pnodes;
-                }
-                ++ran;
-                // one exit: finished lanes, or everyone when enough lanes are
-                // done or the phase limit is reached
-                const uint32_t running = __popcll(__ballot(!fin));
-                if (fin || running <= floor_live || ran >= lim32) {
-                    live = !fin;
-                    break;
-                }
-            }
-            if (!live) {  // account the finished piece
-                exec_nodes += pnodes;
-                acc_nodes += pnodes << ws;
-                acc_sols += (uint64_t)psols << ws;
-            }
-        }
-        // the phase length: the most placements any lane made in it
-        uint64_t rmax = pnodes - pn0;
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint64_t o = __shfl_xor(rmax, off, 64);
-            rmax = o > rmax ? o : rmax;
-        }
-        rmax = rmax ? rmax : 1;  // a phase that only closed pieces still counts
-        pass += rmax;
+            (uint32_t)(limit < (uint64_t)kPhaseCap ? limit : (uint64_t)kPhaseCap));
+        // one (solve, weight) for every lane with a piece: count solutions per
+        // wave (a ballot and a scalar popcount) instead of per lane
+        const uint32_t lead = live_mask ? (uint32_t)__ffsll((long long)live_mask) - 1u : 0u;
+        const uint32_t lead_solve = __shfl(solve, lead, 64);
+        const uint32_t lead_ws = __shfl(ws, lead, 64);
+        const bool uniform = __ballot(top != 0 && (solve != lead_solve || ws != lead_ws)) == 0;
+        uint32_t psols = 0, pnodes = 0;  // this phase (<= kPhaseCap passes: no wrap)
+        uint32_t ran = 0;
+        if (uniform)
+            search_phase<T, true>(st, top, floor_live, lim32, psols, pnodes, ran);
+        else
+            search_phase<T, false>(st, top, floor_live, lim32, psols, pnodes, ran);
+        if (uniform) psols = lane == lead ? psols : 0u;  // the wave's count, once
+        exec_nodes += pnodes;
+        acc_nodes += (uint64_t)pnodes << ws;
+        acc_sols += (uint64_t)psols << ws;
+        pass += ran;
     }
 
     add_per_solve(a.solve_sols, acc_solve, acc_sols, lane);
@@ -423,7 +591,7 @@ hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n) {
 hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
                      Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
                      uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err, const float* pred,
-                     float thresh) {
+                     const float* thresh) {
     hipLaunchKernelGGL(k2_expand_kernel, dim3(grid), dim3(256), 0, s, in, in_count, out, out_count,
                        out_cap, target_row, solve_nodes, exec_nodes, err, pred, thresh);
     return hipGetLastError();
@@ -464,13 +632,31 @@ __global__ void k2_bucket_scan_kernel(uint32_t* __restrict__ hist) {
     }
 }
 
-__global__ void k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ pred,
-                                         const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor,
-                                         Item* __restrict__ out) {
+// One global atomic per (block, bucket): the block counts its items per
+// bucket in LDS, reserves each bucket's range once, then ranks its items
+// inside the range with LDS atomics. Order inside a bucket is arbitrary.
+__global__ void __launch_bounds__(256)
+k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ pred,
+                         const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor,
+                         Item* __restrict__ out) {
+    __shared__ uint32_t h[kBuckets];
+    __shared__ uint32_t off[kBuckets];
     const uint32_t c = *count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
-        const uint32_t o = atomicAdd(&cursor[pred_bucket(pred[i])], 1u);
-        out[o] = in[i];
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < c; b0 += gridDim.x * blockDim.x) {
+        for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
+        __syncthreads();
+        const uint32_t i = b0 + threadIdx.x;
+        uint32_t bk = 0, r = 0;
+        if (i < c) {
+            bk = pred_bucket(pred[i]);
+            r = atomicAdd(&h[bk], 1u);
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < kBuckets; j += blockDim.x)
+            off[j] = h[j] ? atomicAdd(&cursor[j], h[j]) : 0u;
+        __syncthreads();
+        if (i < c) out[off[bk] + r] = in[i];
+        __syncthreads();
     }
 }
 
@@ -481,6 +667,54 @@ hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pr
     hipLaunchKernelGGL(k2_bucket_hist_kernel, dim3(blocks), dim3(256), 0, s, pred, count, hist);
     hipLaunchKernelGGL(k2_bucket_scan_kernel, dim3(1), dim3(64), 0, s, hist);
     hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, hist, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// learned split threshold, computed where the predictions are
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k2_pred_sum_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, double* __restrict__ sum) {
+    __shared__ double part[4];
+    const uint32_t c = *count;
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        v += exp2((double)pred[i]);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) t += part[w];
+        if (t != 0.0) atomicAdd(sum, t);
+    }
+}
+
+__device__ __forceinline__ float split_thresh(float fixed, double scale, const double* sum) {
+    return fixed > 0.f ? fixed : (float)log2(fmax(1.0, *sum * scale));
+}
+
+__global__ void __launch_bounds__(256)
+k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, float fixed,
+                    double scale, const double* __restrict__ sum, uint32_t* __restrict__ over,
+                    float* __restrict__ thresh) {
+    const uint32_t c = *count;
+    const float t = split_thresh(fixed, scale, sum);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *thresh = t;
+    uint32_t n = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        n += pred[i] > t;
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63u) == 0 && n) atomicAdd(over, n);
+}
+
+hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
+                              float fixed, double scale, double* sum, uint32_t* over, float* thresh) {
+    const uint32_t blocks = std::max(1u, std::min(1024u, (max_count + 255u) / 256u));
+    if (!(fixed > 0.f))
+        hipLaunchKernelGGL(k2_pred_sum_kernel, dim3(blocks), dim3(256), 0, s, pred, count, sum);
+    hipLaunchKernelGGL(k2_pred_over_kernel, dim3(blocks), dim3(256), 0, s, pred, count, fixed, scale, sum, over,
+                       thresh);
     return hipGetLastError();
 }
 
@@ -522,7 +756,8 @@ hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* 
 }
 
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads) {
-    return (size_t)stack_rows * threads * sizeof(uint4) + (size_t)threads * sizeof(uint32_t);
+    return threads == 64 ? Stack<64>::bytes(stack_rows)
+                         : threads == 128 ? Stack<128>::bytes(stack_rows) : Stack<256>::bytes(stack_rows);
 }
 
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p) {
@@ -542,19 +777,25 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.donate = p.donate;
     a.pad = 0;
     a.budget = p.budget;
-    if (p.wide)
-        hipLaunchKernelGGL(k2_drain_kernel<true>, dim3(blocks), dim3(threads),
-                           k2_lds_bytes(p.stack_rows, threads), s, a);
+    const size_t lds = k2_lds_bytes(p.stack_rows, threads);
+    if (threads == 64)
+        hipLaunchKernelGGL(k2_drain_kernel<64>, dim3(blocks), dim3(64), lds, s, a);
+    else if (threads == 128)
+        hipLaunchKernelGGL(k2_drain_kernel<128>, dim3(blocks), dim3(128), lds, s, a);
+    else if (threads == 256)
+        hipLaunchKernelGGL(k2_drain_kernel<256>, dim3(blocks), dim3(256), lds, s, a);
     else
-        hipLaunchKernelGGL(k2_drain_kernel<false>, dim3(blocks), dim3(threads),
-                           k2_lds_bytes(p.stack_rows, threads), s, a);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<true>, (int)threads, lds_bytes) !=
-        hipSuccess) {
+    hipError_t e = threads == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<64>, 64, lds_bytes)
+                   : threads == 128
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<128>, 128, lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<256>, 256, lds_bytes);
+    if (e != hipSuccess) {
         (void)hipGetLastError();
         return NQM_EDEVICE;
     }

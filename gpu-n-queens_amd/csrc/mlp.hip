@@ -130,8 +130,9 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
         for (int j = 0; j < 4; ++j) A1[mt][j] = W1[(q4 + j) * kHid + 16 * mt + r16];  // W1^T
         for (int kt = 0; kt < 4; ++kt)
             for (int j = 0; j < 4; ++j) {
-                A2[mt][kt][j] = W2[(16 * kt + q4 + j) * kHid + 16 * mt + r16];   // W2^T
-                A2t[mt][kt][j] = W2[(16 * mt + r16) * kHid + 16 * kt + q4 + j];  // W2
+                A2[mt][kt][j] = W2[(16 * kt + q4 + j) * kHid + 16 * mt + r16];  // W2^T
+                // W2 itself: only the backward pass multiplies by it
+                A2t[mt][kt][j] = a.backward ? W2[(16 * mt + r16) * kHid + 16 * kt + q4 + j] : (_Float16)0.0f;
             }
     }
     float4_t gW2[4][4], gW1[4];
@@ -338,8 +339,12 @@ static int mlp_launch_pass(nqm_mlp* mlp, const _Float16* x, const float* t, cons
     a.loss = static_cast<float*>(mlp->loss.ptr);
     a.backward = backward ? 1 : 0;
     const uint32_t tiles = (count + 15u) / 16u;
-    // backward: a handful of tiles per wave keeps the atomic fold cheap
-    const uint32_t waves = backward ? std::max(1u, std::min(tiles, tiles / 8u + 1u)) : tiles;
+    // backward: a handful of tiles per wave keeps the atomic fold cheap;
+    // forward: enough waves to fill the chip, each loading the weights once
+    // for many tiles
+    const uint32_t fwd_waves = (uint32_t)ctx->prop.multiProcessorCount * 32u;
+    const uint32_t waves = backward ? std::max(1u, std::min(tiles, tiles / 8u + 1u))
+                                    : std::max(1u, std::min(tiles, fwd_waves));
     const uint32_t blocks = std::max(1u, std::min(65535u, (waves + kMlpWaves - 1) / kMlpWaves));
     hipLaunchKernelGGL(mlp_pass_kernel, dim3(blocks), dim3(64 * kMlpWaves), 0, ctx->stream, a);
     NQM_HIP(ctx, hipGetLastError());

@@ -124,6 +124,7 @@ static int count_device_impl(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uin
         st->kernel_ms = ms;
         st->launches = 1;
         st->blocks = plan.blocks;
+        st->search_ms = ms;
     }
     return NQM_OK;
 }
@@ -211,6 +212,10 @@ void nqm_destroy(nqm_ctx* ctx) {
     free_dev(ctx->part_sols);
     free_dev(ctx->part_nodes);
     free_dev(ctx->err);
+    nqm::BatchWork& w = ctx->batch;
+    for (nqm::DevBuf* d : {&w.init, &w.q[0], &w.q[1], &w.counts, &w.head, &w.spill_count, &w.err, &w.target,
+                           &w.sols, &w.nodes, &w.exec, &w.feats, &w.pred, &w.hist, &w.stat})
+        free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

@@ -100,16 +100,6 @@ int build_host_items(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, bool mi
     return NQM_OK;
 }
 
-// Device buffers of one batch call; freed on every exit path.
-struct Batch {
-    DevBuf init, q[2], counts, head, spill_count, err, target, sols, nodes, exec;
-    ~Batch() {
-        for (DevBuf* d : {&init, &q[0], &q[1], &counts, &head, &spill_count, &err, &target, &sols,
-                          &nodes, &exec})
-            if (d->ptr) (void)hipFree(d->ptr);
-    }
-};
-
 // K2 drain grid for a stack depth: as many waves per CU as LDS and the
 // requested cap allow.
 int drain_shape(nqm_ctx* ctx, uint32_t stack_rows, const nqm_tune& t, uint32_t* blocks,
@@ -142,7 +132,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     if (tune_in) t = *tune_in;
     if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH | NQM_TUNE_NO_ORDER))
         return fail(ctx, NQM_EINVAL, "unknown tune flags 0x%x", t.flags);
-    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 16u;
+    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 4u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
     const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
     if ((t.flags & NQM_TUNE_GRAPH) && t.model)
@@ -167,10 +157,12 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         if (t.stack_rows && n - 1 > t.stack_rows + tr) tr = n - 1 - t.stack_rows;
         target[s] = (uint8_t)std::min(tr, n - 1);
     }
-    const uint64_t want_items =
-        auto_depth ? (uint64_t)ctx->prop.multiProcessorCount * 16u * 64u * 8u : UINT64_MAX;
+    // With a model, the plain expansion stops 16x earlier: the learned stage
+    // splits only the items it predicts large.
+    const uint64_t chip_lanes = (uint64_t)ctx->prop.multiProcessorCount * 16u * 64u;
+    const uint64_t want_items = !auto_depth ? UINT64_MAX : t.model ? chip_lanes / 2u : chip_lanes * 8u;
 
-    Batch b;
+    BatchWork& b = ctx->batch;
     const uint32_t count0 = (uint32_t)h.items.size();
     if ((rc = ensure_dev(ctx, b.init, std::max<size_t>(count0, 1) * sizeof(Item))) != NQM_OK) return rc;
     for (DevBuf* d : {&b.head, &b.spill_count, &b.err, &b.exec})
@@ -251,17 +243,18 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     float learn_ms = 0.f;
     uint32_t split_levels = 0;
     if (t.model && count) {
-        const uint32_t rounds = t.split_rounds ? t.split_rounds : 2u;
-        const uint32_t lanes = (uint32_t)ctx->prop.multiProcessorCount * 16u * 64u;
-        DevBuf feats, pred, hist;
-        struct Free {
-            DevBuf* b[3];
-            ~Free() {
-                for (DevBuf* d : b)
-                    if (d->ptr) (void)hipFree(d->ptr);
-            }
-        } fr{{&feats, &pred, &hist}};
+        const uint32_t rounds = t.split_rounds ? t.split_rounds : 3u;
+        const double lanes = (double)ctx->prop.multiProcessorCount * 16.0 * 64.0;
+        DevBuf& feats = b.feats;
+        DevBuf& pred = b.pred;
+        DevBuf& hist = b.hist;
+        DevBuf& stat = b.stat;
         if ((rc = ensure_dev(ctx, hist, 64 * 4)) != NQM_OK) return rc;
+        // stat: double sum of 2^pred, u32 count over the threshold, float threshold
+        if ((rc = ensure_dev(ctx, stat, 32)) != NQM_OK) return rc;
+        double* const d_psum = static_cast<double*>(stat.ptr);
+        uint32_t* const d_over = reinterpret_cast<uint32_t*>(d_psum + 1);
+        float* const d_thresh = reinterpret_cast<float*>(d_over + 1);
         uint32_t* const d_slot = d_cnt + 16;  // slots 16..31 for this stage
         NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
         auto predict = [&](Item* items, uint32_t* cnt_slot, uint32_t n_items) -> int {
@@ -274,19 +267,15 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         for (uint32_t r = 0; r <= rounds; ++r) {
             if ((rc = predict(final_q, final_slot, count)) != NQM_OK) return rc;
             if (r == rounds) break;
-            float thresh = t.split_log2;
-            std::vector<float> hp(count);
-            NQM_HIP(ctx, hipMemcpyAsync(hp.data(), pred.ptr, (size_t)count * 4, hipMemcpyDeviceToHost, st));
+            // split what exceeds 1/8 of a lane's fair share (or the fixed threshold)
+            NQM_HIP(ctx, k2_fill_u32(st, reinterpret_cast<uint32_t*>(d_psum), 4, 0));
+            NQM_HIP(ctx, k2_split_threshold(st, static_cast<const float*>(pred.ptr), final_slot, count,
+                                            t.split_log2, 1.0 / (8.0 * lanes), d_psum, d_over, d_thresh));
+            uint32_t over = 0;
+            NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
-            if (thresh <= 0.f) {  // split what exceeds 1/8 of a lane's fair share
-                double sum = 0;
-                for (float v : hp) sum += std::exp2((double)v);
-                thresh = (float)std::log2(std::max(1.0, sum / (8.0 * lanes)));
-            }
-            uint64_t over = 0;
-            for (float v : hp) over += v > thresh;
             if (over == 0) break;
-            const uint64_t cap64 = (uint64_t)count + over * max_n;
+            const uint64_t cap64 = (uint64_t)count + (uint64_t)over * max_n;
             if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
             if ((rc = ensure_dev(ctx, b.q[spare], (size_t)cap64 * sizeof(Item))) != NQM_OK) return rc;
             uint32_t* out_slot = d_slot + (r & 7);
@@ -294,7 +283,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
             NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, static_cast<Item*>(b.q[spare].ptr), out_slot,
                                    (uint32_t)cap64, d_target, d_nodes, d_exec, d_err,
-                                   static_cast<const float*>(pred.ptr), thresh));
+                                   static_cast<const float*>(pred.ptr), d_thresh));
             NQM_HIP(ctx, hipMemcpyAsync(&count, out_slot, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
             final_q = static_cast<Item*>(b.q[spare].ptr);
@@ -335,7 +324,6 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     p.refill_idle = refill;
     p.donate = (t.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
     p.budget = t.budget;
-    p.wide = max_n >= 19;
     p.head = d_head;
     p.spill_count = d_spill_count;
     p.item_count = d_qcount;
@@ -426,6 +414,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     NQM_HIP(ctx, hipEventElapsedTime(&run_ms, ctx->ev_start, ctx->ev_stop));
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (err & 16u) return fail(ctx, NQM_EDEVICE, "drain LDS does not start at address 0");
     if (err) return fail(ctx, NQM_ERANGE, "queue capacity exceeded (err=0x%x)", err);
 
     uint64_t total = 0;
@@ -442,6 +431,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         stats->wall_ms = now_ms() - t0;
         stats->launches = launches;
         stats->blocks = blocks;
+        stats->search_ms = (double)run_ms;
     }
     (void)rounds;
     (void)split_levels;

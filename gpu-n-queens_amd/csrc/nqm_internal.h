@@ -25,6 +25,15 @@ struct HostBuf {
 
 }  // namespace nqm
 
+namespace nqm {
+// Device buffers of the batch solver, kept on the context so that repeated
+// calls reuse them (they only grow; nqm_destroy frees them).
+struct BatchWork {
+    DevBuf init, q[2], counts, head, spill_count, err, target, sols, nodes, exec;
+    DevBuf feats, pred, hist, stat;  // learned splitting
+};
+}  // namespace nqm
+
 struct nqm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -33,6 +42,7 @@ struct nqm_ctx {
     uint32_t lds_per_block = 0;
     uint32_t lds_per_cu = 0;
     nqm::DevBuf pool, part_sols, part_nodes, err;
+    nqm::BatchWork batch;
     nqm::HostBuf h_part;
     std::string last_error;
 };
@@ -87,14 +97,18 @@ struct K2DrainParams {
     uint32_t refill_idle = 16;
     uint32_t donate = 1;
     uint64_t budget = 0;
-    bool wide = true;  // u64 per-lane solution counters (needed when some n >= 19)
 };
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
 hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
                      Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
                      uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
-                     const float* pred = nullptr, float thresh = 0.f);
+                     const float* pred = nullptr, const float* thresh = nullptr);
+// Split threshold of the learned stage, on the device: thresh = fixed if
+// fixed > 0, else log2(max(1, sum(2^pred) * scale)); over = #(pred > thresh).
+// stats[0..1] (double sum) and stats[2] (u32 over) must be zero on entry.
+hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
+                              float fixed, double scale, double* sum, uint32_t* over, float* thresh);
 hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
                                   uint32_t max_count, uint32_t* hist, Item* out);
 hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,

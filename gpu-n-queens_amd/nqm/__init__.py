@@ -50,7 +50,8 @@ class NqmError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [("total", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
-                ("launches", ctypes.c_uint32), ("blocks", ctypes.c_uint32)]
+                ("launches", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("search_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -49,6 +49,8 @@ typedef struct {
     double wall_ms;        /* host wall time of the whole call */
     uint32_t launches;     /* kernel launches the call made */
     uint32_t blocks;       /* workgroups of the (first) counting launch */
+    double search_ms;      /* device time of the search launches alone (K1 count;
+                              K2 drain rounds), without expansion or splitting */
 } nqm_stats;
 
 /* Create a context on HIP device `device` (its own stream, events and
@@ -195,15 +197,15 @@ typedef struct {
     uint32_t depth;        /* rows placed before the queue (per solve, capped
                               at n-1 and raised to k); 0 = automatic */
     uint32_t waves_per_cu; /* target resident waves per CU; 0 = as many as fit */
-    uint32_t threads;      /* workgroup size 64/128/256; 0 = 256 */
-    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 16 */
+    uint32_t threads;      /* workgroup size 64/128/256; 0 = 64 */
+    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 4 */
     uint64_t budget;       /* passes per wave before live work is spilled and the
                               queue relaunched (restart); 0 = drain in one launch */
     uint32_t flags;        /* NQM_TUNE_* */
     uint32_t stack_rows;   /* LDS stack rows per lane; 0 = what the deepest solve needs */
     /* Learned splitting: with a model, after the expansion every item whose
      * predicted log2(1 + nodes) exceeds split_log2 is split one row deeper,
-     * up to split_rounds times (0 = 2), then the queue is ordered largest
+     * up to split_rounds times (0 = 3), then the queue is ordered largest
      * predicted subtree first. split_log2 = 0 picks log2(sum / (8 x lanes)). */
     struct nqm_mlp* model;
     float split_log2;

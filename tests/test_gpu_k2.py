@@ -45,7 +45,8 @@ def test_c3_mixed_batch_512(ctx):
 @pytest.mark.parametrize("tune", [
     dict(depth=3), dict(depth=5), dict(depth=7),
     dict(waves_per_cu=4), dict(waves_per_cu=8), dict(waves_per_cu=12), dict(waves_per_cu=16),
-    dict(threads=64), dict(threads=128), dict(refill_idle=1), dict(refill_idle=64),
+    dict(threads=64), dict(threads=128), dict(threads=256), dict(refill_idle=1), dict(refill_idle=64),
+    dict(threads=256, budget=300), dict(refill_idle=1, budget=40),
     dict(flags=nqm.TUNE_NO_DONATE), dict(flags=nqm.TUNE_NO_MIRROR), dict(stack_rows=6),
     dict(flags=nqm.TUNE_GRAPH),
 ])
