@@ -149,6 +149,7 @@ k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_co
     uint32_t block_total = 0;
     (void)block_scan256(mine, scratch, &block_total);
     if (threadIdx.x == 0) s_base = block_total ? atomicAdd(out_count, block_total) : 0u;
+    if (!out) return;  // count only: sizes the next level
     __syncthreads();
     const uint32_t obase = s_base;
     const bool fits = (uint64_t)obase + block_total <= out_cap;
@@ -296,14 +297,24 @@ constexpr uint32_t kGrabMax = 256;           // queue items one wave takes per a
 // the inert entry) until at most floor_live lanes hold a piece, or lim32
 // passes ran (checked every kPassUnroll passes). kWaveSols: psols is the
 // wave's solution count (same in every lane), else the lane's own.
+// f2 = a & ~l & ~r in one VALU op (truth table 0x10 over (a, l, r)).
+__device__ __forceinline__ uint32_t free_cells(uint32_t a, uint32_t l, uint32_t r) {
+    uint32_t f;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x10" : "=v"(f) : "v"(a), "v"(l), "v"(r));
+    return f;
+}
+
 template <uint32_t T, bool kWaveSols>
 __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t floor_live, uint32_t lim32,
                                              uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     constexpr uint32_t step = Stack<T>::kSlot;
+    // a lane pops when its current row has no candidate left: the row it
+    // just entered had no free cell, or it returned to an exhausted row
+    bool need_pop = st.x == 0;
     for (;;) {
 #pragma unroll
         for (uint32_t u = 0; u < kPassUnroll; ++u) {
-            if (st.x == 0) {  // row exhausted: resume the nearest open row
+            if (need_pop) {  // resume the nearest open row
                 top = __builtin_elementwise_sub_sat(top, step);
                 st = slot_load<T>(top);
             }
@@ -312,23 +323,25 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t 
             const uint32_t a2 = st.y ^ b;
             const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
             const uint32_t r2 = (st.w | b) >> 1;
-            const uint32_t f2 = a2 & ~l2 & ~r2;
+            const uint32_t f2 = free_cells(a2, l2, r2);
             if (kWaveSols)
                 psols += (uint32_t)__popcll(__ballot(a2 == 0));
             else
                 psols += (a2 == 0);
             // entering a row places each of its candidates once: count them
-            // all now (f2 == 0 when the child is not entered)
+            // all now
             pnodes += (uint32_t)__popc(f2);
-            const bool go = f2 != 0;
-            if (go && st.x != 0) {
+            // Always enter the child: the parent waits on the stack if it has
+            // candidates left, and a child without a free cell is left again
+            // by the next pass's pop (which also places the parent's next
+            // candidate, so a pass is still one placement). Entering needs no
+            // select between parent and child state.
+            if (st.x != 0) {
                 slot_store<T>(top, st);
                 top += step;
             }
-            st.x = go ? f2 : st.x;
-            st.y = go ? a2 : st.y;
-            st.z = go ? l2 : st.z;
-            st.w = go ? r2 : st.w;
+            st = make_uint4(f2, a2, l2, r2);
+            need_pop = f2 == 0;
         }
         ran += kPassUnroll;
         const uint32_t running = __popcll(__ballot(top != 0));

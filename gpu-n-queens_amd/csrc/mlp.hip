@@ -109,6 +109,9 @@ struct MlpArgs {
 
 constexpr int kMlpWaves = 4;  // waves per workgroup
 
+// kBackward = false: prediction only; the gradient state is compiled out,
+// which keeps the kernel small enough for several waves per SIMD.
+template <bool kBackward>
 __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
     // wave-private LDS: h1 [64][16], dh2 [64][16], dh1 [64][16] as f16
     __shared__ _Float16 lds_h1[kMlpWaves][kHid * 16];
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
             for (int j = 0; j < 4; ++j) {
                 A2[mt][kt][j] = W2[(16 * kt + q4 + j) * kHid + 16 * mt + r16];  // W2^T
                 // W2 itself: only the backward pass multiplies by it
-                A2t[mt][kt][j] = a.backward ? W2[(16 * mt + r16) * kHid + 16 * kt + q4 + j] : (_Float16)0.0f;
+                A2t[mt][kt][j] = kBackward ? W2[(16 * mt + r16) * kHid + 16 * kt + q4 + j] : (_Float16)0.0f;
             }
     }
     float4_t gW2[4][4], gW1[4];
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
         part += __shfl_xor(part, 32, 64);
         const float y = part + b3;
         if (a.y && valid && lane < 16) a.y[s] = y;
-        if (!a.backward) continue;
+        if (!kBackward) continue;
 
         // ---- backward ----
         const float err = valid ? (y - a.t[src]) : 0.f;
@@ -247,7 +250,7 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (!a.backward) return;
+    if (!kBackward) return;
 
     // fold this wave's accumulators into the global gradient
     for (int mt = 0; mt < 4; ++mt)
@@ -346,7 +349,10 @@ static int mlp_launch_pass(nqm_mlp* mlp, const _Float16* x, const float* t, cons
     const uint32_t waves = backward ? std::max(1u, std::min(tiles, tiles / 8u + 1u))
                                     : std::max(1u, std::min(tiles, fwd_waves));
     const uint32_t blocks = std::max(1u, std::min(65535u, (waves + kMlpWaves - 1) / kMlpWaves));
-    hipLaunchKernelGGL(mlp_pass_kernel, dim3(blocks), dim3(64 * kMlpWaves), 0, ctx->stream, a);
+    if (backward)
+        hipLaunchKernelGGL(mlp_pass_kernel<true>, dim3(blocks), dim3(64 * kMlpWaves), 0, ctx->stream, a);
+    else
+        hipLaunchKernelGGL(mlp_pass_kernel<false>, dim3(blocks), dim3(64 * kMlpWaves), 0, ctx->stream, a);
     NQM_HIP(ctx, hipGetLastError());
     return NQM_OK;
 }

@@ -100,6 +100,46 @@ int build_host_items(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, bool mi
     return NQM_OK;
 }
 
+// Expand every item below its solve's target row by one row, level by level,
+// while the items stay under max_items (k2_expand on the host).
+constexpr size_t kHostItems = 1u << 14;
+void host_expand(const std::vector<uint8_t>& target, size_t max_items, HostInit* h) {
+    const uint32_t ns = (uint32_t)h->row.size();
+    for (;;) {
+        bool any = false;
+        for (uint32_t s = 0; s < ns; ++s) any |= h->row[s] < target[s];
+        if (!any || h->items.size() >= max_items) return;
+        std::vector<Item> next;
+        next.reserve(h->items.size() * 8);
+        for (const Item& it : h->items) {
+            const uint32_t n = meta_n(it.meta), row = meta_row(it.meta);
+            if (row >= target[it.solve]) {
+                next.push_back(it);
+                continue;
+            }
+            const uint32_t mask = (1u << n) - 1u, ws = meta_wshift(it.meta);
+            const uint32_t placed = (uint32_t)__builtin_popcount(it.f);
+            h->nodes[it.solve] += (uint64_t)placed << ws;
+            h->exec_nodes += placed;
+            for (uint32_t f = it.f; f;) {
+                const uint32_t b = f & (0u - f);
+                f ^= b;
+                Item c{};
+                c.cols = it.cols | b;
+                c.ld = ((it.ld | b) << 1) & mask;
+                c.rd = (it.rd | b) >> 1;
+                c.f = ~(c.cols | c.ld | c.rd) & mask;
+                c.solve = it.solve;
+                c.meta = item_meta(n, row + 1, ws);
+                if (c.f) next.push_back(c);
+            }
+        }
+        for (uint32_t s = 0; s < ns; ++s)
+            if (h->row[s] < target[s]) h->row[s]++;
+        h->items.swap(next);
+    }
+}
+
 // K2 drain grid for a stack depth: as many waves per CU as LDS and the
 // requested cap allow.
 int drain_shape(nqm_ctx* ctx, uint32_t stack_rows, const nqm_tune& t, uint32_t* blocks,
@@ -132,7 +172,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     if (tune_in) t = *tune_in;
     if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH | NQM_TUNE_NO_ORDER))
         return fail(ctx, NQM_EINVAL, "unknown tune flags 0x%x", t.flags);
-    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 4u;
+    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 8u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
     const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
     if ((t.flags & NQM_TUNE_GRAPH) && t.model)
@@ -161,6 +201,10 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     // splits only the items it predicts large.
     const uint64_t chip_lanes = (uint64_t)ctx->prop.multiProcessorCount * 16u * 64u;
     const uint64_t want_items = !auto_depth ? UINT64_MAX : t.model ? chip_lanes / 2u : chip_lanes * 8u;
+
+    // The first levels are tiny: place them on the host (same rule and node
+    // accounting as k2_expand) and save a device round trip per level.
+    host_expand(target, kHostItems, &h);
 
     BatchWork& b = ctx->batch;
     const uint32_t count0 = (uint32_t)h.items.size();
@@ -206,27 +250,44 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     };
     NQM_HIP(ctx, enqueue_clear());
 
-    std::vector<uint32_t> caps;
+    // Level L reads level_in(L) (count in slot L) and writes level_out(L)
+    // (count in slot L+1). Each level is counted first, so its buffer is
+    // allocated exactly. Automatic depth keeps expanding while the queue is
+    // too short to fill the chip, or while the deepest solve would need more
+    // than kRowsGoal LDS stack rows (fewer rows, more resident waves), as long
+    // as a level stays under kAutoMaxItems. With a model the expansion stops
+    // at want_items: the learned stage splits only what it predicts large.
+    constexpr uint32_t kRowsGoal = 9;
+    constexpr uint64_t kAutoMaxItems = 1ull << 27;
+    std::vector<uint32_t> caps, ins;
     std::vector<uint8_t> final_row(h.row);
     uint32_t count = count0;
     float expand_ms = 0.f;
     NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
     for (uint32_t L = 0; L < 11; ++L) {
         bool any = false;
+        uint32_t rows_left = 0;
         for (uint32_t s = 0; s < ns; ++s)
-            if (final_row[s] < target[s]) any = true;
-        if (!any || count == 0 || (uint64_t)count >= want_items) break;
-        const uint64_t cap64 = (uint64_t)count * max_n;
-        if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
-        const uint32_t cap = (uint32_t)cap64;
-        if ((rc = ensure_dev(ctx, b.q[L & 1], (size_t)cap * sizeof(Item))) != NQM_OK) return rc;
+            if (final_row[s] < target[s]) {
+                any = true;
+                rows_left = std::max(rows_left, solves[s].n - 1u - final_row[s]);
+            }
+        if (!any || count == 0) break;
+        if (auto_depth && (uint64_t)count >= want_items && (t.model || rows_left <= kRowsGoal)) break;
         const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, cap,
-                               d_target, d_nodes, d_exec, d_err));
+        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, nullptr, d_cnt + L + 1, 0, d_target, d_nodes,
+                               d_exec, d_err));
         uint32_t next = 0;
         NQM_HIP(ctx, hipMemcpyAsync(&next, d_cnt + L + 1, 4, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
-        caps.push_back(cap);
+        if (auto_depth && next > kAutoMaxItems && (uint64_t)count >= want_items) break;
+        if (next > 0x7fffffffu) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
+        if ((rc = ensure_dev(ctx, b.q[L & 1], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK) return rc;
+        NQM_HIP(ctx, k2_fill_u32(st, d_cnt + L + 1, 1, 0));
+        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, next, d_target,
+                               d_nodes, d_exec, d_err));
+        caps.push_back(next);
+        ins.push_back(count);
         for (uint32_t s = 0; s < ns; ++s)
             if (final_row[s] < target[s]) final_row[s]++;
         count = next;
@@ -342,7 +403,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         NQM_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = enqueue_clear();
         for (uint32_t L = 0; e == hipSuccess && L < levels; ++L) {
-            const uint32_t grid = std::min<uint32_t>(4096u, (caps[L] / max_n + 255u) / 256u);
+            const uint32_t grid = std::min<uint32_t>(4096u, (ins[L] + 255u) / 256u);
             e = k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, caps[L],
                           d_target, d_nodes, d_exec, d_err);
         }

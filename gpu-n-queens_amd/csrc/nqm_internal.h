@@ -100,6 +100,8 @@ struct K2DrainParams {
 };
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
+// One level of children (out == nullptr: only adds the number of children
+// to *out_count, to size the next level).
 hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
                      Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
                      uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,

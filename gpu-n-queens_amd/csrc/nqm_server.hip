@@ -363,7 +363,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         p.exec_nodes = static_cast<uint64_t*>(s->exec.ptr);
         p.err = static_cast<uint32_t*>(s->err.ptr);
         p.stack_rows = rows;
-        p.refill_idle = 4;
+        p.refill_idle = 8;
         p.donate = (s->cfg.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
         p.budget = spill_ok ? budget : 0;
         NQM_HIP(ctx, k2_drain(st, blocks, threads, p));

@@ -198,7 +198,7 @@ typedef struct {
                               at n-1 and raised to k); 0 = automatic */
     uint32_t waves_per_cu; /* target resident waves per CU; 0 = as many as fit */
     uint32_t threads;      /* workgroup size 64/128/256; 0 = 64 */
-    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 4 */
+    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 8 */
     uint64_t budget;       /* passes per wave before live work is spilled and the
                               queue relaunched (restart); 0 = drain in one launch */
     uint32_t flags;        /* NQM_TUNE_* */
