@@ -54,7 +54,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default="all", choices=["all", "plain", "persistent", "learned"])
     ap.add_argument("--epochs", type=int, default=8, help="MLP training epochs (learned variant)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -207,6 +207,17 @@ def workload_n19(ctx, nqm, args):
             "persistent_vs_plain": out["plain"]["kernel_ms"] / out["persistent"]["kernel_ms"]}
 
 
+def _progress(label, every=100):
+    """on_result callback: a line on stderr every `every` completions."""
+    done = [0]
+
+    def cb(tag, rec):
+        done[0] += 1
+        if done[0] % every == 0:
+            print(f"{label}: {done[0]} requests done", file=sys.stderr, flush=True)
+    return cb
+
+
 def workload_serving(ctx, nqm, args):
     from nqm import serving, workloads
     known = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
@@ -215,7 +226,8 @@ def workload_serving(ctx, nqm, args):
     probe = workloads.serving_stream(count=args.requests, rate=1.0)[: max(20, args.requests // 10)]
     srv = ctx.server()
     try:
-        res, wall = serving.replay(srv, [(0.0, n, p) for _, n, p in probe])
+        res, wall = serving.replay(srv, [(0.0, n, p) for _, n, p in probe],
+                                   on_result=_progress("capacity probe", 50))
     finally:
         srv.close()
     capacity = len(probe) / (wall / 1e3)
@@ -223,7 +235,7 @@ def workload_serving(ctx, nqm, args):
     reqs = workloads.serving_stream(count=args.requests, rate=rate)
     srv = ctx.server()
     try:
-        res, wall = serving.replay(srv, reqs)
+        res, wall = serving.replay(srv, reqs, on_result=_progress("stream"))
         rounds = srv.info()["rounds"]
     finally:
         srv.close()
@@ -321,6 +333,7 @@ def main():
         sms = sum(r["search_ms"]) / len(r["search_ms"])
         variants[name] = {
             "path": r["path"], "ms_per_step": ms, "kernel_ms": kms, "search_ms": sms,
+            "step_walls_ms": [round(w, 3) for w in r["walls"]],
             "kernel": r["kernel"],
             "wall_s_to_count": ms / 1e3, "nodes_per_s": HALF_NODES / (ms / 1e3),
             "kernel_nodes": r["kernel_nodes"], "blocks": r["blocks"],

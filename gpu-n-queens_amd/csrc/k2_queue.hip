@@ -23,10 +23,12 @@
 //    its own column mask. Counts are folded per lane and flushed to per-solve
 //    u64 counters when the lane switches solve or exits.
 //
-//  * Optional pass budget: when a wave has run `budget` passes, each live
-//    lane writes its open rows (current row + stack entries) to a spill
-//    queue as independent items, and the wave exits. The host relaunches on
-//    the spill queue (restart), which also lets new work join between rounds.
+//  * Optional pass budget: when a wave has run `budget` passes, each lane
+//    with an unfinished piece parks it (current row + open rows) as one
+//    continuation record, hands back the queue items its wave holds, and the
+//    wave exits. The host relaunches (restart): a later round takes queue
+//    items first (so new work joins between rounds), then resumes the parked
+//    pieces where they stopped.
 
 #include <algorithm>
 #include <type_traits>
@@ -229,8 +231,15 @@ struct DrainArgs {
     uint32_t stack_rows;
     uint32_t refill_idle;
     uint32_t donate;
-    uint32_t pad;
+    uint32_t cont_cap;
     uint64_t budget;  // 0: drain everything
+    uint32_t items_first;
+    uint32_t pad;
+    const Cont* cont_in;
+    const uint32_t* cont_in_count;
+    uint32_t* cont_head;
+    Cont* cont_out;
+    uint32_t* cont_out_count;
 };
 
 __device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, uint64_t& sols,
@@ -291,7 +300,7 @@ __device__ __forceinline__ void slot_store(uint32_t addr, uint4 st) {
 
 constexpr uint32_t kPassUnroll = 4;          // passes between two votes
 constexpr uint32_t kPhaseCap = 1u << 26;     // passes per phase: u32 counters cannot wrap
-constexpr uint32_t kGrabMax = 256;           // queue items one wave takes per atomic
+constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per atomic
 
 // One search phase: every lane places one queen per pass (idle lanes spin on
 // the inert entry) until at most floor_live lanes hold a piece, or lim32
@@ -319,7 +328,7 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t 
                 st = slot_load<T>(top);
             }
             const uint32_t b = st.x & (0u - st.x);
-            st.x ^= b;
+            const uint32_t xr = st.x ^ b;  // the row's other candidates
             const uint32_t a2 = st.y ^ b;
             const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
             const uint32_t r2 = (st.w | b) >> 1;
@@ -336,8 +345,10 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t 
             // by the next pass's pop (which also places the parent's next
             // candidate, so a pass is still one placement). Entering needs no
             // select between parent and child state.
-            if (st.x != 0) {
-                slot_store<T>(top, st);
+            if (xr != 0) {
+                slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
+                // keep the increment after the store: it then updates top in place
+                asm volatile("" : "+v"(top));
                 top += step;
             }
             st = make_uint4(f2, a2, l2, r2);
@@ -383,46 +394,92 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     // and not yet started; rem is what the queue held after the last grab
     bool queue_open = true;
     uint32_t lq = 0, lq_end = 0, rem = count;
+    // parked pieces of earlier rounds (taken after the queue's items)
+    const uint32_t cont_count = a.cont_in ? *a.cont_in_count : 0u;
+    bool cont_open = cont_count != 0;
     uint64_t pass = 0;  // wave-uniform: passes run so far
 
-    for (;;) {
-        // ---- pass budget: spill every open row and every held item, then leave
-        if (a.budget && pass >= a.budget) {
-            if (top != 0) {
-                // open rows: the current one if it has candidates, then the stack
-                const uint32_t cur = st.x != 0 ? 1u : 0u;
-                const uint32_t nitems = cur + (top > bot ? (top - bot) / step : 0u);
-                const uint32_t o = nitems ? atomicAdd(a.spill_count, nitems) : 0u;
-                if (nitems && (uint64_t)o + nitems > a.spill_cap) {
-                    atomicOr(a.err, 4u);
-                } else if (nitems) {
-                    const uint32_t n = 31u - __clz(mask + 1u);
-                    Item it;
-                    it.solve = solve;
-                    it.pad0 = it.pad1 = 0;
-                    uint4 e = st;
-                    uint32_t sa = bot;
-                    uint32_t counted = 0;
-                    for (uint32_t j = 0; j < nitems; ++j) {
-                        if (j || !cur) {
-                            e = slot_load<T>(sa);
-                            sa += step;
-                        }
-                        it.f = e.x;
-                        it.cols = ~e.y & mask;
-                        it.ld = e.z & mask;
-                        it.rd = e.w;
-                        it.meta = item_meta(n, (uint32_t)__popc(it.cols), ws);
-                        a.spill[o + j] = it;
-                        counted += (uint32_t)__popc(e.x);
-                    }
-                    // these candidates were counted when their row was entered;
-                    // the round that drains the spilled items counts them again
-                    exec_nodes -= counted;
-                    acc_nodes -= (uint64_t)counted << ws;
-                }
-                top = 0;
+    // resume parked pieces exactly where they stopped (their nodes are
+    // counted already): current row in VGPRs, open rows to LDS
+    auto resume_parked = [&](uint64_t& idle) {
+        const uint32_t nidle = __popcll(idle);
+        uint32_t cb = 0;
+        if (lane == 0) cb = atomicAdd(a.cont_head, nidle);
+        cb = __shfl(cb, 0, 64);
+        const uint32_t cgot = cb >= cont_count ? 0u : min(nidle, cont_count - cb);
+        if (cgot < nidle) cont_open = false;
+        const uint32_t r = lane_rank(idle);
+        if (top == 0 && !reserved && r < cgot) {
+            const Cont* c = a.cont_in + cb + r;
+            const uint32_t csolve = c->solve, cmeta = c->meta;
+            uint32_t k = c->k;
+            if (k > a.stack_rows) {  // cannot happen with a consistent stack depth
+                atomicOr(a.err, 32u);
+                k = 0;
             }
+            if (csolve != acc_solve) {
+                flush_acc(a, acc_solve, acc_sols, acc_nodes);
+                acc_solve = csolve;
+            }
+            mask = (1u << meta_n(cmeta)) - 1u;
+            solve = csolve;
+            ws = meta_wshift(cmeta);
+            st = *reinterpret_cast<const uint4*>(c->st);
+            for (uint32_t j = 0; j < k; ++j) {
+                const uint4 e = *reinterpret_cast<const uint4*>(c->rows[j]);
+                slot_store<T>(base + j * step, make_uint4(e.w, e.x, e.y, e.z));
+            }
+            bot = base;
+            top = base + k * step;
+        }
+        idle = __ballot(top == 0 && !reserved);
+    };
+
+    for (;;) {
+        // ---- pass budget: park every unfinished piece, hand back every held
+        // item, then leave
+        if (a.budget && pass >= a.budget) {
+            // a piece is unfinished if its current row has a candidate or it
+            // has open rows ([bot, top); below bot are donated slots)
+            const bool park = top != 0 && (st.x != 0 || top > bot);
+            // once the queue and the parked pieces are used up, lanes would
+            // idle next round: a piece with an open row parks its bottom row
+            // (its largest subtree) as a piece of its own, so the next round
+            // spreads the work again
+            const bool split = park && !queue_open && lq == lq_end && !cont_open && top > bot;
+            const uint64_t pm = __ballot(park), sm = __ballot(split);
+            const uint32_t np = (uint32_t)__popcll(pm) + (uint32_t)__popcll(sm);
+            if (np) {
+                uint32_t o = 0;
+                if (lane == 0) o = atomicAdd(a.cont_out_count, np);
+                o = __shfl(o, 0, 64);
+                if ((uint64_t)o + np > a.cont_cap) {
+                    if (lane == 0) atomicOr(a.err, 4u);
+                } else if (park) {
+                    const uint32_t meta = item_meta(31u - __clz(mask + 1u), 0u, ws);
+                    if (split) {
+                        Cont* c = a.cont_out + o + (uint32_t)__popcll(pm) + lane_rank(sm);
+                        c->solve = solve;
+                        c->meta = meta;
+                        c->k = 0;
+                        c->pad = 0;
+                        *reinterpret_cast<uint4*>(c->st) = slot_load<T>(bot);
+                        bot += step;
+                    }
+                    Cont* c = a.cont_out + o + lane_rank(pm);
+                    const uint32_t k = top > bot ? (top - bot) / step : 0u;
+                    c->solve = solve;
+                    c->meta = meta;
+                    c->k = k;
+                    c->pad = 0;
+                    *reinterpret_cast<uint4*>(c->st) = st;
+                    for (uint32_t j = 0; j < k; ++j) {
+                        const uint4 e = slot_load<T>(bot + j * step);
+                        *reinterpret_cast<uint4*>(c->rows[j]) = make_uint4(e.y, e.z, e.w, e.x);
+                    }
+                }
+            }
+            top = 0;
             if (lq < lq_end) {  // taken but never started: back to the queue
                 const uint32_t nloc = lq_end - lq;
                 uint32_t o = 0;
@@ -438,12 +495,14 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
             break;
         }
 
-        // ---- refill idle lanes: held items, then the global queue (one
-        // atomic per grab of up to kGrabMax items), then in-wave donation
+        // ---- refill idle lanes: parked pieces of earlier rounds, then held
+        // items, then the global queue (one atomic per grab of up to kGrabMax
+        // items), then in-wave donation
         uint64_t idle = __ballot(top == 0 && !reserved);
         if (idle) {
+            if (idle && cont_open && !a.items_first) resume_parked(idle);
             const uint32_t nidle = __popcll(idle);
-            if (queue_open || lq < lq_end) {
+            if (idle && (queue_open || lq < lq_end)) {
                 const uint32_t take_local = min(nidle, lq_end - lq);
                 uint32_t g = 0, got = 0;
                 if (take_local < nidle && queue_open) {
@@ -489,6 +548,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                 }
                 idle = __ballot(top == 0 && !reserved);
             }
+            if (idle && cont_open && a.items_first) resume_parked(idle);
             if (idle && a.donate) {
                 const bool is_donor = top > bot;
                 const uint64_t donors = __ballot(is_donor);
@@ -534,7 +594,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
             }
         }
         const uint64_t live_mask = __ballot(top != 0);
-        const bool have_items = queue_open || lq < lq_end;
+        const bool have_items = queue_open || lq < lq_end || cont_open;
         if (live_mask == 0 && !have_items) break;
 
         // ---- search: every lane places one queen per pass (the K1 step;
@@ -731,43 +791,6 @@ hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* 
     return hipGetLastError();
 }
 
-// Items a budget-ended round never handed out: items[head .. count) move to
-// the end of the spill queue so the next round sees them.
-__global__ void k2_append_leftover_kernel(const Item* __restrict__ items,
-                                          const uint32_t* __restrict__ count,
-                                          const uint32_t* __restrict__ head, Item* __restrict__ out,
-                                          uint32_t* __restrict__ out_count, uint32_t out_cap,
-                                          uint32_t* __restrict__ err) {
-    const uint32_t c = *count;
-    const uint32_t h = min(*head, c);
-    const uint32_t left = c - h;
-    const uint32_t o = *out_count;  // stable: the drain round has ended
-    if (left == 0) return;
-    if ((uint64_t)o + left > out_cap) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, 8u);
-        return;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < left; i += gridDim.x * blockDim.x)
-        out[o + i] = items[h + i];
-}
-
-__global__ void k2_bump_count_kernel(const uint32_t* __restrict__ count,
-                                     const uint32_t* __restrict__ head, uint32_t* __restrict__ out_count,
-                                     uint32_t out_cap) {
-    const uint32_t c = *count;
-    const uint32_t h = min(*head, c);
-    if ((uint64_t)*out_count + (c - h) <= out_cap) *out_count += c - h;
-}
-
-hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,
-                              const uint32_t* head, Item* out, uint32_t* out_count, uint32_t out_cap,
-                              uint32_t* err) {
-    hipLaunchKernelGGL(k2_append_leftover_kernel, dim3(1024), dim3(256), 0, s, items, count, head, out,
-                       out_count, out_cap, err);
-    hipLaunchKernelGGL(k2_bump_count_kernel, dim3(1), dim3(1), 0, s, count, head, out_count, out_cap);
-    return hipGetLastError();
-}
-
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads) {
     return threads == 64 ? Stack<64>::bytes(stack_rows)
                          : threads == 128 ? Stack<128>::bytes(stack_rows) : Stack<256>::bytes(stack_rows);
@@ -788,8 +811,15 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.stack_rows = p.stack_rows;
     a.refill_idle = p.refill_idle;
     a.donate = p.donate;
-    a.pad = 0;
     a.budget = p.budget;
+    a.cont_in = p.cont_in;
+    a.cont_in_count = p.cont_in_count;
+    a.cont_head = p.cont_head;
+    a.cont_out = p.cont_out;
+    a.cont_out_count = p.cont_out_count;
+    a.cont_cap = p.cont_cap;
+    a.items_first = p.items_first;
+    a.pad = 0;
     const size_t lds = k2_lds_bytes(p.stack_rows, threads);
     if (threads == 64)
         hipLaunchKernelGGL(k2_drain_kernel<64>, dim3(blocks), dim3(64), lds, s, a);

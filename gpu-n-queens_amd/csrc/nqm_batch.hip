@@ -374,7 +374,6 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     }
     uint32_t blocks = 0, threads = 0;
     if ((rc = drain_shape(ctx, stack_rows, t, &blocks, &threads)) != NQM_OK) return rc;
-    const uint64_t spill_per_round = (uint64_t)blocks * threads * (stack_rows + 1u);
 
     K2DrainParams p;
     p.solve_sols = d_sols;
@@ -428,38 +427,83 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         launches = 1;
         rounds = 1;
     } else {
-        NQM_HIP(ctx, hipMemcpyAsync(d_qcount, final_slot, 4, hipMemcpyDeviceToDevice, st));
         NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        // Budget rounds run in place: a round's spills are appended right
+        // after its queue, and the next round's queue starts where the head
+        // stopped. The live range moves to the front of the other buffer only
+        // when the end of its buffer is near.
         Item* queue = final_q;
-        int spill_buf = spare;
         uint32_t qn = count;
+        int qb = -1;          // b.q[qb] holds the queue (budget rounds)
+        uint64_t qstart = 0;  // its offset there
+        // a round hands back the items its waves hold (the open rows of
+        // unfinished pieces are parked as continuations, one per lane)
+        const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems;
+        const uint64_t lanes = (uint64_t)blocks * threads;
+        uint32_t* d_cc = nullptr;  // in count, head, out count of parked pieces
+        uint32_t conts = 0;        // parked pieces waiting for the next round
+        int ci = 0;                // b.cont[ci] holds them
+        if (t.budget) {
+            if ((rc = ensure_dev(ctx, b.cont_counts, 64)) != NQM_OK) return rc;
+            for (DevBuf* d : {&b.cont[0], &b.cont[1]})
+                if ((rc = ensure_dev(ctx, *d, (size_t)lanes * sizeof(Cont))) != NQM_OK) return rc;
+            d_cc = static_cast<uint32_t*>(b.cont_counts.ptr);
+        }
         for (;;) {
             NQM_HIP(ctx, k2_fill_u32(st, d_head, 1, 0));
-            NQM_HIP(ctx, k2_fill_u32(st, d_spill_count, 1, 0));
-            p.items = queue;
+            NQM_HIP(ctx, k2_fill_u32(st, d_qcount, 1, qn));
             if (t.budget) {
-                const uint64_t cap64 = (uint64_t)qn + spill_per_round;
-                if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "spill queue too large");
-                if ((rc = ensure_dev(ctx, b.q[spill_buf], (size_t)cap64 * sizeof(Item))) != NQM_OK)
+                if (qb < 0 || qstart + qn + spill_room > b.q[qb].cap / sizeof(Item)) {
+                    const int to = qb < 0 ? spare : qb ^ 1;
+                    const uint64_t want = 2 * ((uint64_t)qn + spill_room);
+                    if (want > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "spill queue too large");
+                    if ((rc = ensure_dev(ctx, b.q[to], (size_t)want * sizeof(Item))) != NQM_OK) return rc;
+                    if (qn)
+                        NQM_HIP(ctx, hipMemcpyAsync(b.q[to].ptr, queue, (size_t)qn * sizeof(Item),
+                                                    hipMemcpyDeviceToDevice, st));
+                    queue = static_cast<Item*>(b.q[to].ptr);
+                    qb = to;
+                    qstart = 0;
+                }
+                NQM_HIP(ctx, k2_fill_u32(st, d_spill_count, 1, qn));
+                p.spill = queue;
+                p.spill_cap = (uint32_t)std::min<uint64_t>(b.q[qb].cap / sizeof(Item) - qstart, 0x7fffffffull);
+                // parked pieces: resumed from cont[ci], parked into cont[ci ^ 1]
+                if ((rc = ensure_dev(ctx, b.cont[ci ^ 1], (2 * (size_t)lanes + conts) * sizeof(Cont))) != NQM_OK)
                     return rc;
-                p.spill = static_cast<Item*>(b.q[spill_buf].ptr);
-                p.spill_cap = (uint32_t)cap64;
+                NQM_HIP(ctx, k2_fill_u32(st, d_cc, 3, 0));
+                NQM_HIP(ctx, k2_fill_u32(st, d_cc, 1, conts));
+                p.cont_in = static_cast<const Cont*>(b.cont[ci].ptr);
+                p.cont_in_count = d_cc;
+                p.cont_head = d_cc + 1;
+                p.cont_out = static_cast<Cont*>(b.cont[ci ^ 1].ptr);
+                p.cont_out_count = d_cc + 2;
+                p.cont_cap = (uint32_t)std::min<uint64_t>(b.cont[ci ^ 1].cap / sizeof(Cont), 0x7fffffffull);
             }
+            p.items = queue;
             NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
             ++launches;
             ++rounds;
             if (!t.budget) break;
-            NQM_HIP(ctx, k2_append_leftover(st, queue, d_qcount, d_head, p.spill, d_spill_count,
-                                            p.spill_cap, d_err));
-            uint32_t left = 0;
-            NQM_HIP(ctx, hipMemcpyAsync(&left, d_spill_count, 4, hipMemcpyDeviceToHost, st));
+            uint32_t hd = 0, end = 0, cc[3] = {0, 0, 0};
+            NQM_HIP(ctx, hipMemcpyAsync(&hd, d_head, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipMemcpyAsync(&end, d_spill_count, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipMemcpyAsync(cc, d_cc, 12, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
-            if (left == 0) break;
-            NQM_HIP(ctx, hipMemcpyAsync(d_qcount, d_spill_count, 4, hipMemcpyDeviceToDevice, st));
-            queue = p.spill;
-            qn = left;
-            spill_buf ^= 1;
-            if (rounds > 100000u) return fail(ctx, NQM_ERANGE, "restart rounds do not converge");
+            const uint32_t h0 = std::min(hd, qn);  // items [h0, qn) were never taken
+            queue += h0;
+            qstart += h0;
+            qn = end - h0;
+            // parked pieces no wave resumed this round follow the new ones
+            const uint32_t c0 = std::min(cc[1], conts);
+            if (c0 < conts)
+                NQM_HIP(ctx, hipMemcpyAsync(static_cast<Cont*>(b.cont[ci ^ 1].ptr) + cc[2],
+                                            static_cast<const Cont*>(b.cont[ci].ptr) + c0,
+                                            (size_t)(conts - c0) * sizeof(Cont), hipMemcpyDeviceToDevice, st));
+            conts = cc[2] + (conts - c0);
+            ci ^= 1;
+            if (qn == 0 && conts == 0) break;
+            if (rounds > 1000000u) return fail(ctx, NQM_ERANGE, "restart rounds do not converge");
         }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
     }

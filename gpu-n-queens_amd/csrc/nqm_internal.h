@@ -30,6 +30,7 @@ namespace nqm {
 // calls reuse them (they only grow; nqm_destroy frees them).
 struct BatchWork {
     DevBuf init, q[2], counts, head, spill_count, err, target, sols, nodes, exec;
+    DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
     DevBuf feats, pred, hist, stat;  // learned splitting
 };
 }  // namespace nqm
@@ -82,6 +83,17 @@ hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_
 
 // ---- K2: persistent queue (k2_queue.hip) ----------------------------------
 struct Item;
+// A search piece parked at the end of a budgeted drain round: its current
+// row and its open rows (bottom first, in the LDS slot word order avail, ld,
+// rd, f). A later round resumes it exactly where it stopped, so its nodes
+// are never counted twice and a round never multiplies the work items.
+constexpr uint32_t kContRows = 32;  // more than any stack needs (n <= 31)
+struct Cont {
+    uint32_t solve, meta, k, pad;  // meta = item_meta(n, 0, ws); k open rows
+    uint32_t st[4];                // f, avail, ld, rd of the current row
+    uint32_t rows[kContRows][4];
+};
+
 struct K2DrainParams {
     const Item* items = nullptr;
     const uint32_t* item_count = nullptr;
@@ -97,6 +109,18 @@ struct K2DrainParams {
     uint32_t refill_idle = 16;
     uint32_t donate = 1;
     uint64_t budget = 0;
+    // parked pieces: resumed from cont_in[0, *cont_in_count) (taken through
+    // *cont_head), parked into cont_out (count in *cont_out_count) when the
+    // budget ends the round
+    const Cont* cont_in = nullptr;
+    const uint32_t* cont_in_count = nullptr;
+    uint32_t* cont_head = nullptr;
+    Cont* cont_out = nullptr;
+    uint32_t* cont_out_count = nullptr;
+    uint32_t cont_cap = 0;
+    // 1: refill from the queue before resuming parked pieces (serving: new
+    // requests first); 0: parked pieces first
+    uint32_t items_first = 0;
 };
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
@@ -113,10 +137,10 @@ hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* 
                               float fixed, double scale, double* sum, uint32_t* over, float* thresh);
 hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
                                   uint32_t max_count, uint32_t* hist, Item* out);
-hipError_t k2_append_leftover(hipStream_t s, const Item* items, const uint32_t* count,
-                              const uint32_t* head, Item* out, uint32_t* out_count, uint32_t out_cap,
-                              uint32_t* err);
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
+// Queue items one drain wave takes per atomic on the queue head (at most);
+// a budget round can hand back up to this many unstarted items per wave.
+constexpr uint32_t kGrabItems = 256;
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
 

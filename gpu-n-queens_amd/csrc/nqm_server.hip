@@ -33,11 +33,22 @@ __global__ void srv_zero_slots_kernel(const uint32_t* __restrict__ slots, uint32
     }
 }
 
-__global__ void srv_outstanding_kernel(const Item* __restrict__ items, const uint32_t* __restrict__ count,
-                                       uint32_t* __restrict__ per_slot) {
-    const uint32_t c = *count;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
-        atomicAdd(&per_slot[items[i].solve], 1u);
+// queued[s] = 1 if a parked piece of slot s is waiting.
+__global__ void srv_parked_kernel(const Cont* __restrict__ conts, uint32_t c, uint32_t* __restrict__ queued) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
+        const uint32_t sl = conts[i].solve;
+        if (i == 0 || conts[i - 1].solve != sl) queued[sl] = 1u;
+    }
+}
+
+// queued[s] = 1 if any of items[0..c) belongs to slot s. Only the first item
+// of each run of equal slots stores (items of a request sit together), so
+// the stores are few and need no atomics.
+__global__ void srv_queued_kernel(const Item* __restrict__ items, uint32_t c, uint32_t* __restrict__ queued) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
+        const uint32_t sl = items[i].solve;
+        if (i == 0 || items[i - 1].solve != sl) queued[sl] = 1u;
+    }
 }
 
 }  // namespace nqm
@@ -49,8 +60,11 @@ struct nqm_server {
     uint64_t cap_items = 0;   // admission limit of queued items
     uint64_t buf_items = 0;   // physical capacity of each queue buffer
     nqm::DevBuf q[2], stage[2], counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list,
-        lvl_nodes;
+        lvl_nodes, cont[2], cont_counts;
     int cur = 0;
+    int ci = 0;          // cont[ci] holds the parked pieces
+    uint32_t conts = 0;  // parked pieces waiting for the next round
+    uint64_t qstart = 0;  // the queue is q[cur][qstart, qstart + qcount)
     uint32_t qcount = 0;
     struct Slot {
         bool active = false;
@@ -68,7 +82,8 @@ struct nqm_server {
     uint64_t rounds_total = 0;
     ~nqm_server() {
         for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &counts, &head, &spill_count, &err, &target,
-                               &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes})
+                               &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes, &cont[0], &cont[1],
+                               &cont_counts})
             if (d->ptr) (void)hipFree(d->ptr);
     }
     double clock_ms() const {
@@ -177,6 +192,19 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t roo
     return NQM_OK;
 }
 
+// Make room for `extra` items after the queue: move the live range to the
+// front of the other buffer when the end of this one is near.
+int make_room(nqm_server* s, uint64_t extra) {
+    if (s->qstart + s->qcount + extra <= s->buf_items) return NQM_OK;
+    nqm_ctx* ctx = s->ctx;
+    if (s->qcount)
+        NQM_HIP(ctx, hipMemcpyAsync(s->q[s->cur ^ 1].ptr, static_cast<Item*>(s->q[s->cur].ptr) + s->qstart,
+                                    (size_t)s->qcount * sizeof(Item), hipMemcpyDeviceToDevice, ctx->stream));
+    s->cur ^= 1;
+    s->qstart = 0;
+    return NQM_OK;
+}
+
 uint32_t stack_rows_needed(const nqm_server* s) {
     uint32_t rows = 1;
     for (const auto& sl : s->slots)
@@ -217,7 +245,7 @@ int nqm_server_create(nqm_ctx* ctx, const nqm_server_cfg* cfg, nqm_server** out)
     int rc = NQM_OK;
     for (DevBuf* d : {&s->q[0], &s->q[1]})
         if (rc == NQM_OK) rc = ensure_dev(ctx, *d, s->buf_items * sizeof(Item));
-    for (DevBuf* d : {&s->counts, &s->head, &s->spill_count, &s->err, &s->exec})
+    for (DevBuf* d : {&s->counts, &s->head, &s->spill_count, &s->err, &s->exec, &s->cont_counts})
         if (rc == NQM_OK) rc = ensure_dev(ctx, *d, 64);
     if (rc == NQM_OK) rc = ensure_dev(ctx, s->lvl_nodes, ((size_t)s->slots_n + 1) * 8);
     if (rc == NQM_OK) rc = ensure_dev(ctx, s->target, s->slots_n);
@@ -293,9 +321,16 @@ int nqm_server_submit(nqm_server* s, const nqm_solve* reqs, const uint64_t* tags
         int rc = stage_request(s, q, slot, room, &items, &which);
         if (rc != NQM_OK) return rc;
         if ((uint64_t)s->qcount + items > s->cap_items && s->qcount > 0) break;  // no room: try later
+        // a new request goes in front of the queue when there is room there
+        // (the next round serves it first), else behind it
+        const bool front = items <= s->qstart;
+        if (!front && (rc = make_room(s, items)) != NQM_OK) return rc;
+        Item* const q0 = static_cast<Item*>(s->q[s->cur].ptr);
         if (items)
-            NQM_HIP(ctx, hipMemcpyAsync(static_cast<Item*>(s->q[s->cur].ptr) + s->qcount, s->stage[which].ptr,
-                                        (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice, st));
+            NQM_HIP(ctx, hipMemcpyAsync(front ? q0 + s->qstart - items : q0 + s->qstart + s->qcount,
+                                        s->stage[which].ptr, (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice,
+                                        st));
+        if (front) s->qstart -= items;
         s->qcount += items;
         s->free_slots.pop_back();
         sl.active = true;
@@ -328,7 +363,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
     nqm_ctx* ctx = s->ctx;
     hipStream_t st = ctx->stream;
     int rc;
-    if (s->qcount) {
+    if (s->qcount || s->conts) {
         const uint32_t rows = stack_rows_needed(s);
         nqm_tune t{};
         uint32_t blocks = 0, threads = 0;
@@ -342,22 +377,32 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
             threads = th;
         }
         (void)t;
-        const uint64_t spill_bound = (uint64_t)blocks * threads * (rows + 1u);
+        // a round hands back the unstarted items its waves hold (unfinished
+        // pieces are parked as continuations); the queue itself stays in place
+        const uint64_t spill_bound = (uint64_t)blocks * kGrabItems;
+        const uint64_t lanes = (uint64_t)blocks * threads;
+        if ((rc = ensure_dev(ctx, s->cont[s->ci], std::max<size_t>(s->conts, 1) * sizeof(Cont))) != NQM_OK)
+            return rc;
+        if ((rc = ensure_dev(ctx, s->cont[s->ci ^ 1], (2 * (size_t)lanes + s->conts) * sizeof(Cont))) != NQM_OK)
+            return rc;
+        uint32_t* const d_cc = static_cast<uint32_t*>(s->cont_counts.ptr);
         const uint64_t budget = s->cfg.budget ? s->cfg.budget : 16384u;
+        if ((rc = make_room(s, spill_bound)) != NQM_OK) return rc;
         // with no room for a full round of spills, drain this round to the end
-        const bool spill_ok = (uint64_t)s->qcount + spill_bound <= s->buf_items;
+        const bool spill_ok = s->qstart + (uint64_t)s->qcount + spill_bound <= s->buf_items;
         uint32_t* cnt = static_cast<uint32_t*>(s->counts.ptr);
         uint32_t* qslot = cnt + 8;
         NQM_HIP(ctx, k2_fill_u32(st, qslot, 1, s->qcount));
         NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->head.ptr), 1, 0));
-        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->spill_count.ptr), 1, 0));
+        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->spill_count.ptr), 1, s->qcount));
+        Item* const queue = static_cast<Item*>(s->q[s->cur].ptr) + s->qstart;
         K2DrainParams p;
-        p.items = static_cast<Item*>(s->q[s->cur].ptr);
+        p.items = queue;
         p.item_count = qslot;
         p.head = static_cast<uint32_t*>(s->head.ptr);
-        p.spill = static_cast<Item*>(s->q[s->cur ^ 1].ptr);
+        p.spill = queue;  // spills go right after the queue
         p.spill_count = static_cast<uint32_t*>(s->spill_count.ptr);
-        p.spill_cap = (uint32_t)s->buf_items;
+        p.spill_cap = (uint32_t)std::min<uint64_t>(s->buf_items - s->qstart, 0x7fffffffull);
         p.solve_sols = static_cast<uint64_t*>(s->sols.ptr);
         p.solve_nodes = static_cast<uint64_t*>(s->nodes.ptr);
         p.exec_nodes = static_cast<uint64_t*>(s->exec.ptr);
@@ -366,20 +411,52 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         p.refill_idle = 8;
         p.donate = (s->cfg.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
         p.budget = spill_ok ? budget : 0;
+        // new requests first, unless the parked pieces outnumber the lanes
+        // (then they go first, so that they cannot pile up)
+        p.items_first = s->conts <= lanes ? 1u : 0u;
+        NQM_HIP(ctx, k2_fill_u32(st, d_cc, 3, 0));
+        NQM_HIP(ctx, k2_fill_u32(st, d_cc, 1, s->conts));
+        p.cont_in = static_cast<const Cont*>(s->cont[s->ci].ptr);
+        p.cont_in_count = d_cc;
+        p.cont_head = d_cc + 1;
+        p.cont_out = static_cast<Cont*>(s->cont[s->ci ^ 1].ptr);
+        p.cont_out_count = d_cc + 2;
+        p.cont_cap = (uint32_t)std::min<uint64_t>(s->cont[s->ci ^ 1].cap / sizeof(Cont), 0x7fffffffull);
         NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
-        NQM_HIP(ctx, k2_append_leftover(st, p.items, qslot, p.head, p.spill, p.spill_count, p.spill_cap, p.err));
-        // items still queued per slot
-        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->outst.ptr), s->slots_n, 0));
-        hipLaunchKernelGGL(srv_outstanding_kernel, dim3(1024), dim3(256), 0, st, p.spill, p.spill_count,
-                           static_cast<uint32_t*>(s->outst.ptr));
-        NQM_HIP(ctx, hipGetLastError());
-        uint32_t left = 0, err = 0;
-        NQM_HIP(ctx, hipMemcpyAsync(&left, p.spill_count, 4, hipMemcpyDeviceToHost, st));
+        uint32_t hd = 0, end = 0, err = 0, cc[3] = {0, 0, 0};
+        NQM_HIP(ctx, hipMemcpyAsync(&hd, p.head, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&end, p.spill_count, 4, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipMemcpyAsync(&err, p.err, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(cc, d_cc, 12, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
         if (err) return fail(ctx, NQM_ERANGE, "server queue capacity exceeded (err=0x%x)", err);
-        s->qcount = left;
-        s->cur ^= 1;
+        const uint32_t h0 = std::min(hd, s->qcount);  // items [h0, qcount) were never taken
+        s->qstart += h0;
+        s->qcount = end - h0;
+        // parked pieces no wave resumed this round follow the new ones
+        const uint32_t c0 = std::min(cc[1], s->conts);
+        if (c0 < s->conts)
+            NQM_HIP(ctx, hipMemcpyAsync(static_cast<Cont*>(s->cont[s->ci ^ 1].ptr) + cc[2],
+                                        static_cast<const Cont*>(s->cont[s->ci].ptr) + c0,
+                                        (size_t)(s->conts - c0) * sizeof(Cont), hipMemcpyDeviceToDevice, st));
+        s->conts = cc[2] + (s->conts - c0);
+        s->ci ^= 1;
+        // which slots still have queued work
+        NQM_HIP(ctx, k2_fill_u32(st, static_cast<uint32_t*>(s->outst.ptr), s->slots_n, 0));
+        if (s->conts) {
+            const uint32_t grid = std::min<uint32_t>(4096u, (s->conts + 255u) / 256u);
+            hipLaunchKernelGGL(srv_parked_kernel, dim3(grid), dim3(256), 0, st,
+                               static_cast<const Cont*>(s->cont[s->ci].ptr), s->conts,
+                               static_cast<uint32_t*>(s->outst.ptr));
+            NQM_HIP(ctx, hipGetLastError());
+        }
+        if (s->qcount) {
+            const uint32_t grid = std::min<uint32_t>(4096u, (s->qcount + 255u) / 256u);
+            hipLaunchKernelGGL(srv_queued_kernel, dim3(grid), dim3(256), 0, st,
+                               static_cast<const Item*>(s->q[s->cur].ptr) + s->qstart, s->qcount,
+                               static_cast<uint32_t*>(s->outst.ptr));
+            NQM_HIP(ctx, hipGetLastError());
+        }
         ++s->rounds_total;
     }
     // completed requests: active slots with nothing left in the queue
@@ -395,7 +472,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
             nqm_server::Slot& sl = s->slots[i];
             if (!sl.active) continue;
             ++sl.rounds;
-            if (s->qcount != 0 && outst[i] != 0) continue;
+            if ((s->qcount != 0 || s->conts != 0) && outst[i] != 0) continue;
             nqm_result r{};
             r.tag = sl.tag;
             r.count = hs[i] + sl.host_sols;

@@ -243,9 +243,11 @@ int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
 
 /* ---- serving: continuous batching of count requests ---------------------
  * No reference counterpart. Requests join a device-resident work queue
- * between budgeted drain rounds; a request's result is ready the round its
- * last queued item ends. Admission is by free counter slot and by queue
- * memory. Each count equals that of the request solved alone. */
+ * between budgeted drain rounds (new requests are served first); pieces of
+ * work a round leaves unfinished are parked and resumed by the next round.
+ * A request's result is ready the round its last item or parked piece ends.
+ * Admission is by free counter slot and by queue memory. Each count equals
+ * that of the request solved alone. */
 typedef struct nqm_server nqm_server;
 typedef struct {
     uint32_t max_active;   /* requests in flight (counter slots); 0 = 256 */
