@@ -251,14 +251,16 @@ __device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, ui
 }
 
 // LDS of one drain workgroup of T lanes (T = 64, 128 or 256):
-//   slot k, word j of lane t at 4 (t + 1) + k * 16T + j * 4T bytes: word j of
-//                          one slot is a plane of T consecutive dwords, and the
-//                          slot stride 16T is a multiple of the 64 banks, so a
-//                          wave's access touches 64 consecutive banks whatever
-//                          the stack depth of each lane (no bank conflicts);
-//   the inert entry        at 0, 4T, 8T, 12T: address 0 is below every lane,
-//                          the other three are words 0..2 of slot 0 of lane
-//                          T-1, which is reserved and never holds work;
+//   slot k of lane t       16 bytes in two 8-byte halves: {avail, ld} at
+//                          8 (t + 1) + 16T k and {rd, f} 8T bytes above. Each
+//                          half is a plane of T consecutive 8-byte words and
+//                          the slot stride 16T is a multiple of the 64 banks,
+//                          so one ds_read2st64_b64 / ds_write2st64_b64 moves a
+//                          lane's whole entry without bank conflicts, whatever
+//                          the stack depth of each lane;
+//   the inert entry        at 0 and 8T: address 0 is below every lane, 8T is
+//                          half 0 of slot 0 of lane T-1, which is reserved and
+//                          never holds work;
 //   after rows slots:      64 u32 per wave of donation scratch.
 // The inert entry {f=0, avail=1, ld=~0, rd=2} is a row that is never done:
 // placing its (empty) candidate gives avail 1 (no solution), no child cell
@@ -271,31 +273,28 @@ __device__ __forceinline__ uint4 inert_entry() { return make_uint4(0u, 1u, 0xfff
 
 template <uint32_t T>
 struct Stack {
-    static constexpr uint32_t kPlane = 4u * T;      // bytes per word plane
-    static constexpr uint32_t kSlot = 4u * kPlane;  // bytes per slot (all lanes)
-    // byte address of slot 0 of lane t (word 0)
-    static __device__ __forceinline__ uint32_t base(uint32_t t) { return 4u * (t + 1u); }
+    static constexpr uint32_t kHalf = 8u * T;      // bytes per half plane
+    static constexpr uint32_t kSlot = 2u * kHalf;  // bytes per slot (all lanes)
+    // byte address of slot 0 of lane t (half 0)
+    static __device__ __forceinline__ uint32_t base(uint32_t t) { return 8u * (t + 1u); }
     static size_t bytes(uint32_t rows) { return (size_t)rows * kSlot + (size_t)T * sizeof(uint32_t); }
 };
 
 // Stack slots are addressed by their LDS byte offset; the dynamic LDS of the
-// drain is the only LDS of the kernel, so offset == address. The words of a
-// slot are {avail, ld, rd, f} in planes 0..3.
+// drain is the only LDS of the kernel, so offset == address.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 template <uint32_t T>
 __device__ __forceinline__ uint4 slot_load(uint32_t addr) {
-    const lds_u32* p = reinterpret_cast<const lds_u32*>((size_t)addr);
-    constexpr uint32_t w = Stack<T>::kPlane / 4u;
-    return make_uint4(p[3 * w], p[0], p[w], p[2 * w]);
+    const lds_u64* p = reinterpret_cast<const lds_u64*>((size_t)addr);
+    const uint64_t h0 = p[0], h1 = p[Stack<T>::kHalf / 8u];
+    return make_uint4((uint32_t)(h1 >> 32), (uint32_t)h0, (uint32_t)(h0 >> 32), (uint32_t)h1);
 }
 template <uint32_t T>
 __device__ __forceinline__ void slot_store(uint32_t addr, uint4 st) {
-    lds_u32* p = reinterpret_cast<lds_u32*>((size_t)addr);
-    constexpr uint32_t w = Stack<T>::kPlane / 4u;
-    p[0] = st.y;
-    p[w] = st.z;
-    p[2 * w] = st.w;
-    p[3 * w] = st.x;
+    lds_u64* p = reinterpret_cast<lds_u64*>((size_t)addr);
+    p[0] = (uint64_t)st.y | (uint64_t)st.z << 32;
+    p[Stack<T>::kHalf / 8u] = (uint64_t)st.w | (uint64_t)st.x << 32;
 }
 
 constexpr uint32_t kPassUnroll = 4;          // passes between two votes
@@ -313,9 +312,32 @@ __device__ __forceinline__ uint32_t free_cells(uint32_t a, uint32_t l, uint32_t 
     return f;
 }
 
+// A row that finds its lane's stack full (top == top_end) is handed to the
+// queue as an item instead: its candidates were counted when it was
+// entered, so they go to povf (subtracted again), and the round that takes
+// the item counts them as the item's own.
+__device__ __forceinline__ void overflow_row(const DrainArgs& a, uint4 row, uint32_t mask, uint32_t solve,
+                                          uint32_t ws) {
+    Item it;
+    it.f = row.x;
+    it.cols = ~row.y & mask;
+    it.ld = row.z & mask;
+    it.rd = row.w;
+    it.solve = solve;
+    it.meta = item_meta(31u - __clz(mask + 1u), (uint32_t)__popc(it.cols), ws);
+    it.pad0 = it.pad1 = 0;
+    const uint32_t o = a.spill ? atomicAdd(a.spill_count, 1u) : 0xffffffffu;
+    if (o < a.spill_cap)
+        a.spill[o] = it;
+    else
+        atomicOr(a.err, 8u);
+}
+
 template <uint32_t T, bool kWaveSols>
-__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t floor_live, uint32_t lim32,
-                                             uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
+__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t top_end, uint32_t floor_live,
+                                             uint32_t lim32, uint32_t& psols, uint32_t& pnodes, uint32_t& povf,
+                                             uint32_t& ran, const DrainArgs& a, uint32_t mask, uint32_t solve,
+                                             uint32_t ws) {
     constexpr uint32_t step = Stack<T>::kSlot;
     // a lane pops when its current row has no candidate left: the row it
     // just entered had no free cell, or it returned to an exhausted row
@@ -338,18 +360,23 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t 
             else
                 psols += (a2 == 0);
             // entering a row places each of its candidates once: count them
-            // all now
-            pnodes += (uint32_t)__popc(f2);
+            // all now (one accumulating v_bcnt)
+            asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(pnodes) : "v"(f2));
             // Always enter the child: the parent waits on the stack if it has
             // candidates left, and a child without a free cell is left again
             // by the next pass's pop (which also places the parent's next
             // candidate, so a pass is still one placement). Entering needs no
             // select between parent and child state.
             if (xr != 0) {
-                slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
-                // keep the increment after the store: it then updates top in place
-                asm volatile("" : "+v"(top));
-                top += step;
+                if (top != top_end) {
+                    slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
+                    // keep the increment after the store: it then updates top in place
+                    asm volatile("" : "+v"(top));
+                    top += step;
+                } else {  // stack full (rare): the row goes to the queue
+                    overflow_row(a, make_uint4(xr, st.y, st.z, st.w), mask, solve, ws);
+                    povf += (uint32_t)__popc(xr);
+                }
             }
             st = make_uint4(f2, a2, l2, r2);
             need_pop = f2 == 0;
@@ -368,6 +395,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
     const uint32_t base = Stack<T>::base(tid);  // > 0: 0 means "no piece"
+    const uint32_t top_end = base + a.stack_rows * step;  // one past the lane's last slot
     const bool reserved = tid == T - 1u;        // its slot 0 holds the inert entry
     uint32_t* const wscratch =
         reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (size_t)a.stack_rows * step) + wave * 64u;
@@ -620,15 +648,19 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         const uint32_t lead_solve = __shfl(solve, lead, 64);
         const uint32_t lead_ws = __shfl(ws, lead, 64);
         const bool uniform = __ballot(top != 0 && (solve != lead_solve || ws != lead_ws)) == 0;
-        uint32_t psols = 0, pnodes = 0;  // this phase (<= kPhaseCap passes: no wrap)
+        uint32_t psols = 0, pnodes = 0, povf = 0;  // this phase (<= kPhaseCap passes: no wrap)
         uint32_t ran = 0;
         if (uniform)
-            search_phase<T, true>(st, top, floor_live, lim32, psols, pnodes, ran);
+            search_phase<T, true>(st, top, top_end, floor_live, lim32, psols, pnodes, povf, ran, a, mask, solve,
+                                  ws);
         else
-            search_phase<T, false>(st, top, floor_live, lim32, psols, pnodes, ran);
+            search_phase<T, false>(st, top, top_end, floor_live, lim32, psols, pnodes, povf, ran, a, mask, solve,
+                                   ws);
         if (uniform) psols = lane == lead ? psols : 0u;  // the wave's count, once
         exec_nodes += pnodes;
+        exec_nodes -= povf;
         acc_nodes += (uint64_t)pnodes << ws;
+        acc_nodes -= (uint64_t)povf << ws;
         acc_sols += (uint64_t)psols << ws;
         pass += ran;
     }

@@ -304,7 +304,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     float learn_ms = 0.f;
     uint32_t split_levels = 0;
     if (t.model && count) {
-        const uint32_t rounds = t.split_rounds ? t.split_rounds : 3u;
+        const uint32_t rounds = t.split_rounds ? t.split_rounds : 1u;
         const double lanes = (double)ctx->prop.multiProcessorCount * 16.0 * 64.0;
         DevBuf& feats = b.feats;
         DevBuf& pred = b.pred;
@@ -361,6 +361,44 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             final_q = static_cast<Item*>(b.q[spare].ptr);
             spare ^= 1;
         }
+        // Then the stack-depth goal of the automatic depth: every item is
+        // expanded to row n-1-kRowsGoal (split items may be deeper already).
+        // Expansion keeps the queue's order, so the largest-predicted-first
+        // order carries over to the children.
+        if (auto_depth) {
+            std::vector<uint8_t> goal(ns);
+            for (uint32_t s = 0; s < ns; ++s) {
+                const uint32_t n = solves[s].n;
+                const uint32_t g = n > kRowsGoal + 1u ? n - 1u - kRowsGoal : 0u;
+                goal[s] = (uint8_t)std::max<uint32_t>(final_row[s], std::min(g, n - 1u));
+            }
+            NQM_HIP(ctx, hipMemcpyAsync(b.target.ptr, goal.data(), ns, hipMemcpyHostToDevice, st));
+            for (uint32_t L = 0;; ++L) {
+                bool below = false;
+                for (uint32_t s = 0; s < ns; ++s) below |= final_row[s] < goal[s];
+                if (!below || count == 0) break;
+                uint32_t* out_slot = d_cnt + 24 + (L & 3);
+                const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
+                NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
+                NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, nullptr, out_slot, 0, d_target, d_nodes,
+                                       d_exec, d_err));
+                uint32_t next = 0;
+                NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
+                NQM_HIP(ctx, hipStreamSynchronize(st));
+                if (next > kAutoMaxItems) break;
+                if ((rc = ensure_dev(ctx, b.q[spare], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK)
+                    return rc;
+                NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
+                NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, static_cast<Item*>(b.q[spare].ptr),
+                                       out_slot, next, d_target, d_nodes, d_exec, d_err));
+                final_q = static_cast<Item*>(b.q[spare].ptr);
+                final_slot = out_slot;
+                spare ^= 1;
+                count = next;
+                for (uint32_t s = 0; s < ns; ++s)
+                    if (final_row[s] < goal[s]) final_row[s]++;
+            }
+        }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
         NQM_HIP(ctx, hipEventElapsedTime(&learn_ms, ctx->ev_start, ctx->ev_stop));
@@ -372,6 +410,12 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         const uint32_t n = solves[s].n;
         if (n > final_row[s] + 1) stack_rows = std::max(stack_rows, n - 1 - final_row[s]);
     }
+    // With an automatic stack depth, a deep stack gets one row less than the
+    // deepest piece could need: a row that finds its lane's stack full goes
+    // to the queue as an item (rare: from depth-8 items at N=18, 1 push in
+    // 10^4 reaches the last row), and the freed LDS holds more waves per CU.
+    const bool capped = t.stack_rows == 0 && !use_graph && stack_rows >= 8;
+    if (capped) --stack_rows;
     uint32_t blocks = 0, threads = 0;
     if ((rc = drain_shape(ctx, stack_rows, t, &blocks, &threads)) != NQM_OK) return rc;
 
@@ -434,12 +478,16 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         // when the end of its buffer is near.
         Item* queue = final_q;
         uint32_t qn = count;
-        int qb = -1;          // b.q[qb] holds the queue (budget rounds)
-        uint64_t qstart = 0;  // its offset there
+        // b.q[qb] holds the queue (rounds in place), at offset qstart
+        int qb = final_q == b.q[0].ptr ? 0 : final_q == b.q[1].ptr ? 1 : -1;
+        uint64_t qstart = 0;
         // a round hands back the items its waves hold (the open rows of
-        // unfinished pieces are parked as continuations, one per lane)
-        const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems;
+        // unfinished pieces are parked as continuations, one per lane) and the
+        // rows that found a full stack
         const uint64_t lanes = (uint64_t)blocks * threads;
+        const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems +
+                                    (capped ? std::max<uint64_t>(4 * lanes, count / 16u) : 0u);
+        const bool rounds_in_place = t.budget || capped;
         uint32_t* d_cc = nullptr;  // in count, head, out count of parked pieces
         uint32_t conts = 0;        // parked pieces waiting for the next round
         int ci = 0;                // b.cont[ci] holds them
@@ -452,7 +500,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         for (;;) {
             NQM_HIP(ctx, k2_fill_u32(st, d_head, 1, 0));
             NQM_HIP(ctx, k2_fill_u32(st, d_qcount, 1, qn));
-            if (t.budget) {
+            if (rounds_in_place) {
                 if (qb < 0 || qstart + qn + spill_room > b.q[qb].cap / sizeof(Item)) {
                     const int to = qb < 0 ? spare : qb ^ 1;
                     const uint64_t want = 2 * ((uint64_t)qn + spill_room);
@@ -468,6 +516,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
                 NQM_HIP(ctx, k2_fill_u32(st, d_spill_count, 1, qn));
                 p.spill = queue;
                 p.spill_cap = (uint32_t)std::min<uint64_t>(b.q[qb].cap / sizeof(Item) - qstart, 0x7fffffffull);
+            }
+            if (t.budget) {
                 // parked pieces: resumed from cont[ci], parked into cont[ci ^ 1]
                 if ((rc = ensure_dev(ctx, b.cont[ci ^ 1], (2 * (size_t)lanes + conts) * sizeof(Cont))) != NQM_OK)
                     return rc;
@@ -484,11 +534,11 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             NQM_HIP(ctx, k2_drain(st, blocks, threads, p));
             ++launches;
             ++rounds;
-            if (!t.budget) break;
+            if (!rounds_in_place) break;
             uint32_t hd = 0, end = 0, cc[3] = {0, 0, 0};
             NQM_HIP(ctx, hipMemcpyAsync(&hd, d_head, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipMemcpyAsync(&end, d_spill_count, 4, hipMemcpyDeviceToHost, st));
-            NQM_HIP(ctx, hipMemcpyAsync(cc, d_cc, 12, hipMemcpyDeviceToHost, st));
+            if (t.budget) NQM_HIP(ctx, hipMemcpyAsync(cc, d_cc, 12, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
             const uint32_t h0 = std::min(hd, qn);  // items [h0, qn) were never taken
             queue += h0;

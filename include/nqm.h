@@ -199,14 +199,19 @@ typedef struct {
     uint32_t waves_per_cu; /* target resident waves per CU; 0 = as many as fit */
     uint32_t threads;      /* workgroup size 64/128/256; 0 = 64 */
     uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 8 */
-    uint64_t budget;       /* passes per wave before live work is spilled and the
-                              queue relaunched (restart); 0 = drain in one launch */
+    uint64_t budget;       /* passes per wave before unfinished pieces are parked
+                              and the drain relaunched (restart); 0 = one launch */
     uint32_t flags;        /* NQM_TUNE_* */
-    uint32_t stack_rows;   /* LDS stack rows per lane; 0 = what the deepest solve needs */
-    /* Learned splitting: with a model, after the expansion every item whose
-     * predicted log2(1 + nodes) exceeds split_log2 is split one row deeper,
-     * up to split_rounds times (0 = 3), then the queue is ordered largest
-     * predicted subtree first. split_log2 = 0 picks log2(sum / (8 x lanes)). */
+    uint32_t stack_rows;   /* LDS stack rows per lane (the expansion then goes deep
+                              enough that no piece needs more); 0 = automatic: what
+                              the deepest piece needs, one less from 8 rows on (a row
+                              that finds its stack full is queued as an item) */
+    /* Learned splitting: with a model, the expansion stops early, every item
+     * whose predicted log2(1 + nodes) exceeds split_log2 is split one row
+     * deeper, up to split_rounds times (0 = 1), the queue is ordered largest
+     * predicted subtree first, and (automatic depth) the expansion then
+     * continues to the stack-depth goal keeping that order. split_log2 = 0
+     * picks log2(sum / (8 x lanes)). */
     struct nqm_mlp* model;
     float split_log2;
     uint32_t split_rounds;
