@@ -7,8 +7,12 @@
 //   2. k2_expand grows items level by level on the device until each solve
 //      reaches its target row (explicit depth) or the queue holds enough
 //      items to fill the chip (automatic depth);
-//   3. k2_drain empties the queue; with a pass budget, live work is spilled
-//      and the drain relaunched until nothing is left (restart);
+//   3. k2_drain empties the queue. With a pass budget, a wave that runs out
+//      parks its live stack as a continuation and hands back the items it
+//      still holds; with a capped LDS stack, a lane that would overflow
+//      writes that row back as an item. Both land right after the queue in
+//      the same buffer, and the drain is relaunched over them (continuations
+//      first) until nothing is left (restart);
 //   4. per-solve u64 counters come back in one copy.
 // With NQM_TUNE_GRAPH, steps 2-4 are captured once into a HIP graph (the
 // level sizes learned by a first, uncaptured expansion) and launched as one

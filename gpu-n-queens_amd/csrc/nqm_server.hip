@@ -8,10 +8,14 @@
 //
 //   submit: admit while a counter slot is free and the request's expanded
 //           items fit the queue memory; the request is expanded on the
-//           device (staging buffer) and appended to the queue tail.
-//   step:   one drain round (pass budget) -> unfinished work is spilled to
-//           the other queue buffer with the never-started items; a
-//           histogram of items per slot says which requests are done.
+//           device (staging buffer) and put in front of the queue when
+//           there is room before it (new requests start first), else at
+//           the tail.
+//   step:   one drain round (pass budget) over parked continuations and
+//           queued items; waves out of budget park their stacks as new
+//           continuations, held items go back right after the queue (in
+//           place), and flag kernels over what is left say which requests
+//           are done.
 //   poll:   finished results (count, nodes, admission and completion clock).
 // Counts per request are exactly those of nqm_solve_batch on that request
 // alone: items are disjoint subtrees, and counters are per slot.
