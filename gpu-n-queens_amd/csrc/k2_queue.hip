@@ -297,55 +297,37 @@ __device__ __forceinline__ void slot_store(uint32_t addr, uint4 st) {
     p[Stack<T>::kHalf / 8u] = (uint64_t)st.w | (uint64_t)st.x << 32;
 }
 
-constexpr uint32_t kPassUnroll = 4;          // passes between two votes
+#ifndef NQM_PASS_UNROLL
+#define NQM_PASS_UNROLL 8
+#endif
+constexpr uint32_t kPassUnroll = NQM_PASS_UNROLL;  // passes between two votes
 constexpr uint32_t kPhaseCap = 1u << 26;     // passes per phase: u32 counters cannot wrap
 constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per atomic
 
 // One search phase: every lane places one queen per pass (idle lanes spin on
 // the inert entry) until at most floor_live lanes hold a piece, or lim32
 // passes ran (checked every kPassUnroll passes). kWaveSols: psols is the
-// wave's solution count (same in every lane), else the lane's own.
-// f2 = a & ~l & ~r in one VALU op (truth table 0x10 over (a, l, r)).
-__device__ __forceinline__ uint32_t free_cells(uint32_t a, uint32_t l, uint32_t r) {
-    uint32_t f;
-    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x10" : "=v"(f) : "v"(a), "v"(l), "v"(r));
-    return f;
-}
-
-// A row that finds its lane's stack full (top == top_end) is handed to the
-// queue as an item instead: its candidates were counted when it was
-// entered, so they go to povf (subtracted again), and the round that takes
-// the item counts them as the item's own.
-__device__ __forceinline__ void overflow_row(const DrainArgs& a, uint4 row, uint32_t mask, uint32_t solve,
-                                          uint32_t ws) {
-    Item it;
-    it.f = row.x;
-    it.cols = ~row.y & mask;
-    it.ld = row.z & mask;
-    it.rd = row.w;
-    it.solve = solve;
-    it.meta = item_meta(31u - __clz(mask + 1u), (uint32_t)__popc(it.cols), ws);
-    it.pad0 = it.pad1 = 0;
-    const uint32_t o = a.spill ? atomicAdd(a.spill_count, 1u) : 0xffffffffu;
-    if (o < a.spill_cap)
-        a.spill[o] = it;
-    else
-        atomicOr(a.err, 8u);
-}
-
+// wave's solution count (same in every lane), else the lane's own. pnodes
+// gains the candidates of every row entered (placements made below it).
 template <uint32_t T, bool kWaveSols>
-__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t top_end, uint32_t floor_live,
-                                             uint32_t lim32, uint32_t& psols, uint32_t& pnodes, uint32_t& povf,
-                                             uint32_t& ran, const DrainArgs& a, uint32_t mask, uint32_t solve,
-                                             uint32_t ws) {
+__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t floor_live, uint32_t lim32,
+                                             uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     constexpr uint32_t step = Stack<T>::kSlot;
-    // a lane pops when its current row has no candidate left: the row it
-    // just entered had no free cell, or it returned to an exhausted row
-    bool need_pop = st.x == 0;
+    // A row's candidates are counted at the start of the pass after it was
+    // entered, while st.x still holds them (a lane that pops has st.x == 0,
+    // so the pop can overwrite st.x in place). The row held at entry was
+    // counted already, the row held at exit not yet.
+    pnodes -= (uint32_t)__popc(st.x);
     for (;;) {
 #pragma unroll
         for (uint32_t u = 0; u < kPassUnroll; ++u) {
-            if (need_pop) {  // resume the nearest open row
+            pnodes += (uint32_t)__popc(st.x);
+            // pins the count here: deferred, it would need a copy of st.x
+            asm volatile("" ::"v"(pnodes));
+            // a lane pops when its current row has no candidate left: the row
+            // it just entered had no free cell, or it returned to an
+            // exhausted row; it resumes the nearest open row
+            if (st.x == 0) {
                 top = __builtin_elementwise_sub_sat(top, step);
                 st = slot_load<T>(top);
             }
@@ -354,37 +336,28 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t 
             const uint32_t a2 = st.y ^ b;
             const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
             const uint32_t r2 = (st.w | b) >> 1;
-            const uint32_t f2 = free_cells(a2, l2, r2);
+            // f2 = a2 & ~l2 & ~r2 in one op (truth table 0x10 over its inputs)
+            const uint32_t f2 = __builtin_amdgcn_bitop3_b32(a2, l2, r2, 0x10);
             if (kWaveSols)
                 psols += (uint32_t)__popcll(__ballot(a2 == 0));
             else
                 psols += (a2 == 0);
-            // entering a row places each of its candidates once: count them
-            // all now (one accumulating v_bcnt)
-            asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(pnodes) : "v"(f2));
             // Always enter the child: the parent waits on the stack if it has
             // candidates left, and a child without a free cell is left again
             // by the next pass's pop (which also places the parent's next
             // candidate, so a pass is still one placement). Entering needs no
             // select between parent and child state.
             if (xr != 0) {
-                if (top != top_end) {
-                    slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
-                    // keep the increment after the store: it then updates top in place
-                    asm volatile("" : "+v"(top));
-                    top += step;
-                } else {  // stack full (rare): the row goes to the queue
-                    overflow_row(a, make_uint4(xr, st.y, st.z, st.w), mask, solve, ws);
-                    povf += (uint32_t)__popc(xr);
-                }
+                slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
+                top += step;
             }
             st = make_uint4(f2, a2, l2, r2);
-            need_pop = f2 == 0;
         }
         ran += kPassUnroll;
         const uint32_t running = __popcll(__ballot(top != 0));
         if (running <= floor_live || ran >= lim32) break;
     }
+    pnodes += (uint32_t)__popc(st.x);
 }
 
 template <uint32_t T>
@@ -395,7 +368,6 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
     const uint32_t base = Stack<T>::base(tid);  // > 0: 0 means "no piece"
-    const uint32_t top_end = base + a.stack_rows * step;  // one past the lane's last slot
     const bool reserved = tid == T - 1u;        // its slot 0 holds the inert entry
     uint32_t* const wscratch =
         reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (size_t)a.stack_rows * step) + wave * 64u;
@@ -648,19 +620,15 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         const uint32_t lead_solve = __shfl(solve, lead, 64);
         const uint32_t lead_ws = __shfl(ws, lead, 64);
         const bool uniform = __ballot(top != 0 && (solve != lead_solve || ws != lead_ws)) == 0;
-        uint32_t psols = 0, pnodes = 0, povf = 0;  // this phase (<= kPhaseCap passes: no wrap)
+        uint32_t psols = 0, pnodes = 0;  // this phase (<= kPhaseCap passes: no wrap)
         uint32_t ran = 0;
         if (uniform)
-            search_phase<T, true>(st, top, top_end, floor_live, lim32, psols, pnodes, povf, ran, a, mask, solve,
-                                  ws);
+            search_phase<T, true>(st, top, floor_live, lim32, psols, pnodes, ran);
         else
-            search_phase<T, false>(st, top, top_end, floor_live, lim32, psols, pnodes, povf, ran, a, mask, solve,
-                                   ws);
+            search_phase<T, false>(st, top, floor_live, lim32, psols, pnodes, ran);
         if (uniform) psols = lane == lead ? psols : 0u;  // the wave's count, once
         exec_nodes += pnodes;
-        exec_nodes -= povf;
         acc_nodes += (uint64_t)pnodes << ws;
-        acc_nodes -= (uint64_t)povf << ws;
         acc_sols += (uint64_t)psols << ws;
         pass += ran;
     }

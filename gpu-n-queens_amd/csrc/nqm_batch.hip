@@ -9,10 +9,9 @@
 //      items to fill the chip (automatic depth);
 //   3. k2_drain empties the queue. With a pass budget, a wave that runs out
 //      parks its live stack as a continuation and hands back the items it
-//      still holds; with a capped LDS stack, a lane that would overflow
-//      writes that row back as an item. Both land right after the queue in
-//      the same buffer, and the drain is relaunched over them (continuations
-//      first) until nothing is left (restart);
+//      still holds (right after the queue, in the same buffer), and the
+//      drain is relaunched over them (continuations first) until nothing is
+//      left (restart);
 //   4. per-solve u64 counters come back in one copy.
 // With NQM_TUNE_GRAPH, steps 2-4 are captured once into a HIP graph (the
 // level sizes learned by a first, uncaptured expansion) and launched as one
@@ -414,12 +413,6 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         const uint32_t n = solves[s].n;
         if (n > final_row[s] + 1) stack_rows = std::max(stack_rows, n - 1 - final_row[s]);
     }
-    // With an automatic stack depth, a deep stack gets one row less than the
-    // deepest piece could need: a row that finds its lane's stack full goes
-    // to the queue as an item (rare: from depth-8 items at N=18, 1 push in
-    // 10^4 reaches the last row), and the freed LDS holds more waves per CU.
-    const bool capped = t.stack_rows == 0 && !use_graph && stack_rows >= 8;
-    if (capped) --stack_rows;
     uint32_t blocks = 0, threads = 0;
     if ((rc = drain_shape(ctx, stack_rows, t, &blocks, &threads)) != NQM_OK) return rc;
 
@@ -486,12 +479,10 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         int qb = final_q == b.q[0].ptr ? 0 : final_q == b.q[1].ptr ? 1 : -1;
         uint64_t qstart = 0;
         // a round hands back the items its waves hold (the open rows of
-        // unfinished pieces are parked as continuations, one per lane) and the
-        // rows that found a full stack
+        // unfinished pieces are parked as continuations, one per lane)
         const uint64_t lanes = (uint64_t)blocks * threads;
-        const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems +
-                                    (capped ? std::max<uint64_t>(4 * lanes, count / 16u) : 0u);
-        const bool rounds_in_place = t.budget || capped;
+        const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems;
+        const bool rounds_in_place = t.budget != 0;
         uint32_t* d_cc = nullptr;  // in count, head, out count of parked pieces
         uint32_t conts = 0;        // parked pieces waiting for the next round
         int ci = 0;                // b.cont[ci] holds them
