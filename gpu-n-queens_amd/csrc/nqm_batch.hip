@@ -212,20 +212,19 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     BatchWork& b = ctx->batch;
     const uint32_t count0 = (uint32_t)h.items.size();
     if ((rc = ensure_dev(ctx, b.init, std::max<size_t>(count0, 1) * sizeof(Item))) != NQM_OK) return rc;
-    for (DevBuf* d : {&b.head, &b.spill_count, &b.err, &b.exec})
+    for (DevBuf* d : {&b.head, &b.spill_count})
         if ((rc = ensure_dev(ctx, *d, 64)) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.counts, 32 * 4)) != NQM_OK) return rc;  // 32 count slots
     if ((rc = ensure_dev(ctx, b.target, std::max<size_t>(ns, 1))) != NQM_OK) return rc;
-    if ((rc = ensure_dev(ctx, b.sols, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
-    if ((rc = ensure_dev(ctx, b.nodes, std::max<size_t>(ns, 1) * 8)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.res, 16 + (size_t)ns * 16)) != NQM_OK) return rc;
 
     uint32_t* const d_cnt = static_cast<uint32_t*>(b.counts.ptr);  // 32 slots
     uint32_t* const d_head = static_cast<uint32_t*>(b.head.ptr);
     uint32_t* const d_spill_count = static_cast<uint32_t*>(b.spill_count.ptr);
-    uint32_t* const d_err = static_cast<uint32_t*>(b.err.ptr);
-    uint64_t* const d_exec = static_cast<uint64_t*>(b.exec.ptr);
-    uint64_t* const d_sols = static_cast<uint64_t*>(b.sols.ptr);
-    uint64_t* const d_nodes = static_cast<uint64_t*>(b.nodes.ptr);
+    uint64_t* const d_exec = static_cast<uint64_t*>(b.res.ptr);
+    uint32_t* const d_err = reinterpret_cast<uint32_t*>(d_exec + 1);
+    uint64_t* const d_sols = d_exec + 2;
+    uint64_t* const d_nodes = d_sols + ns;
     const uint8_t* const d_target = static_cast<uint8_t*>(b.target.ptr);
     uint32_t* const d_qcount = d_cnt + 12;  // the drain reads its queue size here
     hipStream_t st = ctx->stream;
@@ -246,10 +245,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         hipError_t e;
         if ((e = k2_fill_u32(st, d_cnt, 16, 0)) != hipSuccess) return e;
         if ((e = k2_fill_u32(st, d_cnt, 1, count0)) != hipSuccess) return e;
-        if ((e = k2_fill_u32(st, d_err, 1, 0)) != hipSuccess) return e;
-        if ((e = k2_fill_u64(st, d_exec, 1)) != hipSuccess) return e;
-        if ((e = k2_fill_u64(st, d_sols, ns)) != hipSuccess) return e;
-        return k2_fill_u64(st, d_nodes, ns);
+        return k2_fill_u64(st, d_exec, 2 + 2 * ns);  // exec, err, sols, nodes
     };
     NQM_HIP(ctx, enqueue_clear());
 
@@ -553,14 +549,13 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
     }
 
-    std::vector<uint64_t> hs(ns), hn(ns);
-    uint64_t exec = 0;
-    uint32_t err = 0;
-    NQM_HIP(ctx, hipMemcpyAsync(hs.data(), d_sols, ns * 8, hipMemcpyDeviceToHost, st));
-    NQM_HIP(ctx, hipMemcpyAsync(hn.data(), d_nodes, ns * 8, hipMemcpyDeviceToHost, st));
-    NQM_HIP(ctx, hipMemcpyAsync(&exec, d_exec, 8, hipMemcpyDeviceToHost, st));
-    NQM_HIP(ctx, hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint64_t> res(2 + 2 * (size_t)ns);
+    NQM_HIP(ctx, hipMemcpyAsync(res.data(), b.res.ptr, res.size() * 8, hipMemcpyDeviceToHost, st));
     NQM_HIP(ctx, hipStreamSynchronize(st));
+    const uint64_t exec = res[0];
+    const uint32_t err = (uint32_t)res[1];
+    const uint64_t* const hs = res.data() + 2;
+    const uint64_t* const hn = hs + ns;
     NQM_HIP(ctx, hipEventElapsedTime(&run_ms, ctx->ev_start, ctx->ev_stop));
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);

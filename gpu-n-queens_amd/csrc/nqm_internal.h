@@ -29,7 +29,9 @@ namespace nqm {
 // Device buffers of the batch solver, kept on the context so that repeated
 // calls reuse them (they only grow; nqm_destroy frees them).
 struct BatchWork {
-    DevBuf init, q[2], counts, head, spill_count, err, target, sols, nodes, exec;
+    // res: u64 executed nodes, u32 error bits, u32 pad, then per solve u64
+    // solutions and u64 nodes (one copy brings every result back)
+    DevBuf init, q[2], counts, head, spill_count, target, res;
     DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
     DevBuf feats, pred, hist, stat;  // learned splitting
 };
