@@ -80,10 +80,18 @@ __device__ __forceinline__ void add_per_solve(uint64_t* __restrict__ arr, uint32
 // expand: one level of children for every item below its solve's target row
 // ---------------------------------------------------------------------------
 // Block b owns the contiguous chunk [b*c, (b+1)*c) of the input, c = ceil(cnt
-// / gridDim). Pass 1 counts the chunk's children, one atomic reserves the
-// chunk's output range, pass 2 writes the children in input order. Node
-// counts are folded per thread and flushed once. Same-address atomics are
-// therefore per block, not per wave or item.
+// / gridDim). The count pass stores each block's number of children and its
+// placements (a per-block record, no atomics); one block sums the level
+// size. The write pass places each block's children after those of the
+// blocks ranked before it, and one block then adds the placements to the
+// per-solve counters. Same-address atomics, which serialize across the
+// chip, are left to blocks that hold items of several solves.
+//
+// Rank = block index keeps input order (the learned stage's largest-first
+// order carries over to the children). Rank = the bit-reversed block index
+// interleaves the input's regions: a drain then works on many solves and on
+// big and small subtrees at once, instead of sweeping one solve's items at a
+// time (all lanes updating that solve's counters together).
 
 // Which input items expand, and into how many live children.
 struct ExpandRule {
@@ -128,36 +136,117 @@ __device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t* scratch,
     return before + incl - x;
 }
 
-__global__ void __launch_bounds__(256)
-k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
-                 Item* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t out_cap,
-                 const uint8_t* __restrict__ target_row, uint64_t* __restrict__ solve_nodes,
-                 uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err,
-                 const float* __restrict__ pred, const float* __restrict__ thresh_p) {
-    __shared__ uint32_t scratch[4];
-    __shared__ uint32_t s_base;
-    const uint32_t cnt = *in_count;
-    const ExpandRule rule{target_row, pred, pred ? *thresh_p : 0.f};
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t chunk = (cnt + gridDim.x - 1u) / gridDim.x;
-    const uint32_t c0 = min(cnt, blockIdx.x * chunk), c1 = min(cnt, c0 + chunk);
+// Block-wide sum of one u64 per thread (blockDim = 256), returned to every
+// thread. Ends with the scratch free for reuse.
+__device__ __forceinline__ uint64_t block_sum256_u64(uint64_t x, uint64_t* scratch) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    x = wave_sum64(x);
+    if (lane == 0u) scratch[wave] = x;
+    __syncthreads();
+    const uint64_t sum = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+    __syncthreads();
+    return sum;
+}
 
-    // pass 1: children of the chunk
+// Per-block record of one level: children, and the block's placements when
+// all its expanded items belong to one solve (solve == kMixed: the write
+// pass adds them itself, per wave).
+struct ExpandPart {
+    uint32_t children, solve;
+    uint64_t nodes, exec;  // weighted and executed placements
+    uint64_t pad;
+};
+static_assert(sizeof(ExpandPart) == 32, "kExpandPartBytes assumes 32-byte records");
+constexpr uint32_t kMixed = 0xffffffffu;
+
+__device__ __forceinline__ void expand_chunk(uint32_t cnt, uint32_t* c0, uint32_t* c1) {
+    const uint32_t chunk = (cnt + gridDim.x - 1u) / gridDim.x;
+    *c0 = min(cnt, blockIdx.x * chunk);
+    *c1 = min(cnt, *c0 + chunk);
+}
+
+__global__ void __launch_bounds__(256)
+k2_expand_count_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
+                       ExpandPart* __restrict__ part, const uint8_t* __restrict__ target_row,
+                       const float* __restrict__ pred, const float* __restrict__ thresh_p) {
+    __shared__ uint32_t scratch[4];
+    __shared__ uint64_t scratch64[4];
+    __shared__ uint32_t s_mixed;
+    const ExpandRule rule{target_row, pred, pred ? *thresh_p : 0.f};
+    uint32_t c0, c1;
+    expand_chunk(*in_count, &c0, &c1);
+    if (threadIdx.x == 0) s_mixed = 0;
+    __syncthreads();
+    const uint32_t solve0 = c0 < c1 ? in[c0].solve : 0u;
     uint32_t mine = 0;
+    uint64_t nodes = 0, exec = 0;
+    bool mixed = false;
     for (uint32_t i = c0 + threadIdx.x; i < c1; i += 256u) {
         const Item it = in[i];
-        mine += rule.expands(it, i) ? live_children(it) : 1u;
+        if (rule.expands(it, i)) {
+            mine += live_children(it);
+            const uint32_t placed = (uint32_t)__popc(it.f);
+            nodes += (uint64_t)placed << meta_wshift(it.meta);
+            exec += placed;
+            mixed |= it.solve != solve0;
+        } else {
+            mine += 1u;
+        }
     }
-    uint32_t block_total = 0;
-    (void)block_scan256(mine, scratch, &block_total);
-    if (threadIdx.x == 0) s_base = block_total ? atomicAdd(out_count, block_total) : 0u;
-    if (!out) return;  // count only: sizes the next level
-    __syncthreads();
-    const uint32_t obase = s_base;
-    const bool fits = (uint64_t)obase + block_total <= out_cap;
-    if (!fits && threadIdx.x == 0) atomicOr(err, 2u);
+    if (mixed) s_mixed = 1u;  // benign race: every writer stores 1
+    uint32_t total = 0;
+    (void)block_scan256(mine, scratch, &total);
+    nodes = block_sum256_u64(nodes, scratch64);
+    exec = block_sum256_u64(exec, scratch64);
+    if (threadIdx.x == 0) {
+        ExpandPart r;
+        r.children = total;
+        r.solve = s_mixed ? kMixed : solve0;
+        r.nodes = nodes;
+        r.exec = exec;
+        r.pad = 0;
+        part[blockIdx.x] = r;
+    }
+}
 
-    // pass 2: write in input order; count the placements of expanded items
+// One block: *out_count = the level size (saturated at 2^32 - 1, which the
+// host rejects).
+__global__ void __launch_bounds__(256)
+k2_expand_total_kernel(const ExpandPart* __restrict__ part, uint32_t nblocks, uint32_t* __restrict__ out_count) {
+    __shared__ uint64_t scratch64[4];
+    uint64_t mine = 0;
+    for (uint32_t j = threadIdx.x; j < nblocks; j += 256u) mine += part[j].children;
+    const uint64_t total = block_sum256_u64(mine, scratch64);
+    if (threadIdx.x == 0) *out_count = (uint32_t)min(total, (uint64_t)0xffffffffu);
+}
+
+__global__ void __launch_bounds__(256)
+k2_expand_write_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_count,
+                       const ExpandPart* __restrict__ part, uint32_t interleave, Item* __restrict__ out,
+                       uint32_t out_cap, const uint8_t* __restrict__ target_row, uint64_t* __restrict__ solve_nodes,
+                       uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err,
+                       const float* __restrict__ pred, const float* __restrict__ thresh_p) {
+    __shared__ uint32_t scratch[4];
+    const ExpandRule rule{target_row, pred, pred ? *thresh_p : 0.f};
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t c0, c1;
+    expand_chunk(*in_count, &c0, &c1);
+    // this block's first output: the children of the blocks ranked before it
+    uint32_t before = 0;
+    if (interleave) {
+        const uint32_t key = __brev(blockIdx.x);
+        for (uint32_t j = threadIdx.x; j < gridDim.x; j += 256u)
+            before += __brev(j) < key ? part[j].children : 0u;
+    } else {
+        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 256u) before += part[j].children;
+    }
+    uint32_t obase = 0;
+    (void)block_scan256(before, scratch, &obase);
+    const ExpandPart mine_part = part[blockIdx.x];
+    const bool fits = (uint64_t)obase + mine_part.children <= out_cap;
+    if (!fits && threadIdx.x == 0) atomicOr(err, 2u);
+    const bool own_nodes = mine_part.solve == kMixed;  // else added by the commit kernel
+
     uint32_t acc_solve = 0;
     uint64_t acc_nodes = 0, acc_exec = 0;
     uint32_t o = obase;
@@ -175,15 +264,14 @@ k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_co
         uint32_t w = o + block_scan256(nch, scratch, &step_total);
         o += step_total;
         if (i >= c1) continue;
-        if (expand) {
+        if (expand && own_nodes) {
             const uint32_t placed = (uint32_t)__popc(it.f);
-            const uint32_t ws = meta_wshift(it.meta);
             if (it.solve != acc_solve && acc_nodes) {
                 atomicAdd((unsigned long long*)&solve_nodes[acc_solve], (unsigned long long)acc_nodes);
                 acc_nodes = 0;
             }
             acc_solve = it.solve;
-            acc_nodes += (uint64_t)placed << ws;
+            acc_nodes += (uint64_t)placed << meta_wshift(it.meta);
             acc_exec += placed;
         }
         if (!fits) continue;
@@ -208,6 +296,35 @@ k2_expand_kernel(const Item* __restrict__ in, const uint32_t* __restrict__ in_co
             c.pad0 = c.pad1 = 0;
             if (c.f) out[w++] = c;
         }
+    }
+    if (own_nodes) {  // block-uniform
+        add_per_solve(solve_nodes, acc_solve, acc_nodes, lane);
+        const uint64_t wexec = wave_sum64(acc_exec);
+        if (lane == 0 && wexec) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wexec);
+    }
+}
+
+// One block: the placements of the single-solve blocks of a written level.
+// Consecutive blocks mostly share a solve, so each thread sums a run of
+// blocks per solve and the waves add them up (a few atomics per level).
+__global__ void __launch_bounds__(256)
+k2_expand_commit_kernel(const ExpandPart* __restrict__ part, uint32_t nblocks,
+                        uint64_t* __restrict__ solve_nodes, uint64_t* __restrict__ exec_nodes) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t per = (nblocks + 255u) / 256u;
+    const uint32_t j0 = min(nblocks, threadIdx.x * per), j1 = min(nblocks, j0 + per);
+    uint32_t acc_solve = 0;
+    uint64_t acc_nodes = 0, acc_exec = 0;
+    for (uint32_t j = j0; j < j1; ++j) {
+        const ExpandPart r = part[j];
+        if (r.solve == kMixed || r.exec == 0) continue;
+        if (r.solve != acc_solve && acc_nodes) {
+            atomicAdd((unsigned long long*)&solve_nodes[acc_solve], (unsigned long long)acc_nodes);
+            acc_nodes = 0;
+        }
+        acc_solve = r.solve;
+        acc_nodes += r.nodes;
+        acc_exec += r.exec;
     }
     add_per_solve(solve_nodes, acc_solve, acc_nodes, lane);
     const uint64_t wexec = wave_sum64(acc_exec);
@@ -242,12 +359,29 @@ struct DrainArgs {
     uint32_t* cont_out_count;
 };
 
-__device__ __forceinline__ void flush_acc(const DrainArgs& a, uint32_t solve, uint64_t& sols,
-                                          uint64_t& nodes) {
-    if (sols) atomicAdd((unsigned long long*)&a.solve_sols[solve], (unsigned long long)sols);
-    if (nodes) atomicAdd((unsigned long long*)&a.solve_nodes[solve], (unsigned long long)nodes);
-    sols = 0;
-    nodes = 0;
+// Lanes that take a piece of another solve (sw) hand over what they counted
+// for the old one. Lanes handing over the same solve are summed first: one
+// atomic per solve and wave, not per lane (lanes of a wave usually change
+// solve together). All lanes must call.
+__device__ __forceinline__ void switch_solve(const DrainArgs& a, bool sw, uint32_t new_solve, uint32_t& acc_solve,
+                                             uint64_t& sols, uint64_t& nodes, uint32_t lane) {
+    bool pend = sw && (sols | nodes) != 0;
+    for (uint64_t m = __ballot(pend); m; m = __ballot(pend)) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+        const uint32_t s = __shfl(acc_solve, leader, 64);
+        const bool mine = pend && acc_solve == s;
+        const uint64_t ts = wave_sum64(mine ? sols : 0u), tn = wave_sum64(mine ? nodes : 0u);
+        if (lane == leader) {
+            if (ts) atomicAdd((unsigned long long*)&a.solve_sols[s], (unsigned long long)ts);
+            if (tn) atomicAdd((unsigned long long*)&a.solve_nodes[s], (unsigned long long)tn);
+        }
+        if (mine) {
+            sols = 0;
+            nodes = 0;
+            pend = false;
+        }
+    }
+    if (sw) acc_solve = new_solve;
 }
 
 // LDS of one drain workgroup of T lanes (T = 64, 128 or 256):
@@ -409,17 +543,16 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         const uint32_t cgot = cb >= cont_count ? 0u : min(nidle, cont_count - cb);
         if (cgot < nidle) cont_open = false;
         const uint32_t r = lane_rank(idle);
-        if (top == 0 && !reserved && r < cgot) {
-            const Cont* c = a.cont_in + cb + r;
-            const uint32_t csolve = c->solve, cmeta = c->meta;
+        const bool take = top == 0 && !reserved && r < cgot;
+        const Cont* const c = a.cont_in + cb + r;
+        const uint32_t csolve = take ? c->solve : 0u;
+        switch_solve(a, take && csolve != acc_solve, csolve, acc_solve, acc_sols, acc_nodes, lane);
+        if (take) {
+            const uint32_t cmeta = c->meta;
             uint32_t k = c->k;
             if (k > a.stack_rows) {  // cannot happen with a consistent stack depth
                 atomicOr(a.err, 32u);
                 k = 0;
-            }
-            if (csolve != acc_solve) {
-                flush_acc(a, acc_solve, acc_sols, acc_nodes);
-                acc_solve = csolve;
             }
             mask = (1u << meta_n(cmeta)) - 1u;
             solve = csolve;
@@ -525,12 +658,11 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     else if (r - take_local < got)
                         idx = g + (r - take_local);
                 }
-                if (idx != 0xffffffffu) {
-                    const Item it = a.items[idx];
-                    if (it.solve != acc_solve) {
-                        flush_acc(a, acc_solve, acc_sols, acc_nodes);
-                        acc_solve = it.solve;
-                    }
+                const bool take = idx != 0xffffffffu;
+                Item it{};
+                if (take) it = a.items[idx];
+                switch_solve(a, take && it.solve != acc_solve, it.solve, acc_solve, acc_sols, acc_nodes, lane);
+                if (take) {
                     mask = (1u << meta_n(it.meta)) - 1u;
                     st = make_uint4(it.f, ~it.cols & mask, it.ld, it.rd);
                     solve = it.solve;
@@ -567,15 +699,12 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     const uint32_t d_mask = __shfl(mask, dl, 64);
                     const uint32_t d_solve = __shfl(solve, dl, 64);
                     const uint32_t d_ws = __shfl(ws, dl, 64);
+                    switch_solve(a, recv && d_solve != acc_solve, d_solve, acc_solve, acc_sols, acc_nodes, lane);
                     if (recv) {
                         // the donor's shallowest open row, straight from its LDS
                         // slot; its candidates were counted when the donor
                         // entered that row
                         st = slot_load<T>(d_bot);
-                        if (d_solve != acc_solve) {
-                            flush_acc(a, acc_solve, acc_sols, acc_nodes);
-                            acc_solve = d_solve;
-                        }
                         mask = d_mask;
                         solve = d_solve;
                         ws = d_ws;
@@ -661,12 +790,26 @@ hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n) {
     return hipGetLastError();
 }
 
-hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
-                     Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
-                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err, const float* pred,
-                     const float* thresh) {
-    hipLaunchKernelGGL(k2_expand_kernel, dim3(grid), dim3(256), 0, s, in, in_count, out, out_count,
-                       out_cap, target_row, solve_nodes, exec_nodes, err, pred, thresh);
+hipError_t k2_expand_count(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count, void* part,
+                           uint32_t* out_count, const uint8_t* target_row, const float* pred,
+                           const float* thresh) {
+    if (grid == 0 || grid > kExpandMaxGrid) return hipErrorInvalidValue;
+    ExpandPart* const p = static_cast<ExpandPart*>(part);
+    hipLaunchKernelGGL(k2_expand_count_kernel, dim3(grid), dim3(256), 0, s, in, in_count, p, target_row, pred,
+                       thresh);
+    hipLaunchKernelGGL(k2_expand_total_kernel, dim3(1), dim3(256), 0, s, p, grid, out_count);
+    return hipGetLastError();
+}
+
+hipError_t k2_expand_write(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
+                           const void* part, bool interleave, Item* out, uint32_t out_cap,
+                           const uint8_t* target_row, uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
+                           const float* pred, const float* thresh) {
+    if (grid == 0 || grid > kExpandMaxGrid) return hipErrorInvalidValue;
+    const ExpandPart* const p = static_cast<const ExpandPart*>(part);
+    hipLaunchKernelGGL(k2_expand_write_kernel, dim3(grid), dim3(256), 0, s, in, in_count, p,
+                       interleave ? 1u : 0u, out, out_cap, target_row, solve_nodes, exec_nodes, err, pred, thresh);
+    hipLaunchKernelGGL(k2_expand_commit_kernel, dim3(1), dim3(256), 0, s, p, grid, solve_nodes, exec_nodes);
     return hipGetLastError();
 }
 

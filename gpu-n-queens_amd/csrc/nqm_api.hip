@@ -214,7 +214,7 @@ void nqm_destroy(nqm_ctx* ctx) {
     free_dev(ctx->err);
     nqm::BatchWork& w = ctx->batch;
     for (nqm::DevBuf* d : {&w.init, &w.q[0], &w.q[1], &w.counts, &w.head, &w.spill_count, &w.target, &w.res,
-                           &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred, &w.hist, &w.stat})
+                           &w.part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred, &w.hist, &w.stat})
         free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);

@@ -217,6 +217,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     if ((rc = ensure_dev(ctx, b.counts, 32 * 4)) != NQM_OK) return rc;  // 32 count slots
     if ((rc = ensure_dev(ctx, b.target, std::max<size_t>(ns, 1))) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.res, 16 + (size_t)ns * 16)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.part, kExpandPartBytes)) != NQM_OK) return rc;
 
     uint32_t* const d_cnt = static_cast<uint32_t*>(b.counts.ptr);  // 32 slots
     uint32_t* const d_head = static_cast<uint32_t*>(b.head.ptr);
@@ -227,6 +228,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     uint64_t* const d_nodes = d_sols + ns;
     const uint8_t* const d_target = static_cast<uint8_t*>(b.target.ptr);
     uint32_t* const d_qcount = d_cnt + 12;  // the drain reads its queue size here
+    void* const d_part = b.part.ptr;        // per-block records of the level being expanded
     hipStream_t st = ctx->stream;
 
     NQM_HIP(ctx, hipMemcpyAsync(b.target.ptr, target.data(), ns, hipMemcpyHostToDevice, st));
@@ -273,18 +275,16 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             }
         if (!any || count == 0) break;
         if (auto_depth && (uint64_t)count >= want_items && (t.model || rows_left <= kRowsGoal)) break;
-        const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, nullptr, d_cnt + L + 1, 0, d_target, d_nodes,
-                               d_exec, d_err));
+        const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
+        NQM_HIP(ctx, k2_expand_count(st, grid, level_in(L), d_cnt + L, d_part, d_cnt + L + 1, d_target));
         uint32_t next = 0;
         NQM_HIP(ctx, hipMemcpyAsync(&next, d_cnt + L + 1, 4, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
         if (auto_depth && next > kAutoMaxItems && (uint64_t)count >= want_items) break;
         if (next > 0x7fffffffu) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
         if ((rc = ensure_dev(ctx, b.q[L & 1], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK) return rc;
-        NQM_HIP(ctx, k2_fill_u32(st, d_cnt + L + 1, 1, 0));
-        NQM_HIP(ctx, k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, next, d_target,
-                               d_nodes, d_exec, d_err));
+        NQM_HIP(ctx, k2_expand_write(st, grid, level_in(L), d_cnt + L, d_part, !t.model, level_out(L), next,
+                                     d_target, d_nodes, d_exec, d_err));
         caps.push_back(next);
         ins.push_back(count);
         for (uint32_t s = 0; s < ns; ++s)
@@ -339,11 +339,13 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
             if ((rc = ensure_dev(ctx, b.q[spare], (size_t)cap64 * sizeof(Item))) != NQM_OK) return rc;
             uint32_t* out_slot = d_slot + (r & 7);
-            NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
-            const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-            NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, static_cast<Item*>(b.q[spare].ptr), out_slot,
-                                   (uint32_t)cap64, d_target, d_nodes, d_exec, d_err,
-                                   static_cast<const float*>(pred.ptr), d_thresh));
+            const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
+            const float* const d_pred = static_cast<const float*>(pred.ptr);
+            NQM_HIP(ctx, k2_expand_count(st, grid, final_q, final_slot, d_part, out_slot, d_target, d_pred,
+                                         d_thresh));
+            NQM_HIP(ctx, k2_expand_write(st, grid, final_q, final_slot, d_part, false,
+                                         static_cast<Item*>(b.q[spare].ptr), (uint32_t)cap64, d_target, d_nodes,
+                                         d_exec, d_err, d_pred, d_thresh));
             NQM_HIP(ctx, hipMemcpyAsync(&count, out_slot, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
             final_q = static_cast<Item*>(b.q[spare].ptr);
@@ -377,19 +379,17 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
                 for (uint32_t s = 0; s < ns; ++s) below |= final_row[s] < goal[s];
                 if (!below || count == 0) break;
                 uint32_t* out_slot = d_cnt + 24 + (L & 3);
-                const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-                NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
-                NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, nullptr, out_slot, 0, d_target, d_nodes,
-                                       d_exec, d_err));
+                const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
+                NQM_HIP(ctx, k2_expand_count(st, grid, final_q, final_slot, d_part, out_slot, d_target));
                 uint32_t next = 0;
                 NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
                 NQM_HIP(ctx, hipStreamSynchronize(st));
                 if (next > kAutoMaxItems) break;
                 if ((rc = ensure_dev(ctx, b.q[spare], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK)
                     return rc;
-                NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
-                NQM_HIP(ctx, k2_expand(st, grid, final_q, final_slot, static_cast<Item*>(b.q[spare].ptr),
-                                       out_slot, next, d_target, d_nodes, d_exec, d_err));
+                NQM_HIP(ctx, k2_expand_write(st, grid, final_q, final_slot, d_part, false,
+                                             static_cast<Item*>(b.q[spare].ptr), next, d_target, d_nodes, d_exec,
+                                             d_err));
                 final_q = static_cast<Item*>(b.q[spare].ptr);
                 final_slot = out_slot;
                 spare ^= 1;
@@ -439,9 +439,11 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         NQM_HIP(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t e = enqueue_clear();
         for (uint32_t L = 0; e == hipSuccess && L < levels; ++L) {
-            const uint32_t grid = std::min<uint32_t>(4096u, (ins[L] + 255u) / 256u);
-            e = k2_expand(st, grid, level_in(L), d_cnt + L, level_out(L), d_cnt + L + 1, caps[L],
-                          d_target, d_nodes, d_exec, d_err);
+            const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (ins[L] + 255u) / 256u);
+            e = k2_expand_count(st, grid, level_in(L), d_cnt + L, d_part, d_cnt + L + 1, d_target);
+            if (e == hipSuccess)
+                e = k2_expand_write(st, grid, level_in(L), d_cnt + L, d_part, true, level_out(L), caps[L],
+                                    d_target, d_nodes, d_exec, d_err);
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st);

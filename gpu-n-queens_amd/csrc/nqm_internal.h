@@ -31,7 +31,7 @@ namespace nqm {
 struct BatchWork {
     // res: u64 executed nodes, u32 error bits, u32 pad, then per solve u64
     // solutions and u64 nodes (one copy brings every result back)
-    DevBuf init, q[2], counts, head, spill_count, target, res;
+    DevBuf init, q[2], counts, head, spill_count, target, res, part;
     DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
     DevBuf feats, pred, hist, stat;  // learned splitting
 };
@@ -126,12 +126,25 @@ struct K2DrainParams {
 };
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
-// One level of children (out == nullptr: only adds the number of children
-// to *out_count, to size the next level).
-hipError_t k2_expand(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
-                     Item* out, uint32_t* out_count, uint32_t out_cap, const uint8_t* target_row,
-                     uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
-                     const float* pred = nullptr, const float* thresh = nullptr);
+// One level of children in two steps, with no same-address atomics (each
+// one would serialize the blocks of the level):
+//   k2_expand_count: per-block child counts and placements into `part`
+//     (kExpandPartBytes), and the level size into *out_count;
+//   k2_expand_write: the children, then the placements of the level added
+//     to solve_nodes / exec_nodes. Children keep input order, or, with
+//     interleave, the blocks' runs of children are interleaved (bit-reversed
+//     block order) so that a queue mixes the regions of its input.
+// A write needs the count of the same input, grid and rule just before it
+// on the stream; a level that is counted but not written adds nothing.
+constexpr uint32_t kExpandMaxGrid = 4096;
+constexpr size_t kExpandPartBytes = (size_t)kExpandMaxGrid * 32u;
+hipError_t k2_expand_count(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count, void* part,
+                           uint32_t* out_count, const uint8_t* target_row, const float* pred = nullptr,
+                           const float* thresh = nullptr);
+hipError_t k2_expand_write(hipStream_t s, uint32_t grid, const Item* in, const uint32_t* in_count,
+                           const void* part, bool interleave, Item* out, uint32_t out_cap,
+                           const uint8_t* target_row, uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
+                           const float* pred = nullptr, const float* thresh = nullptr);
 // Split threshold of the learned stage, on the device: thresh = fixed if
 // fixed > 0, else log2(max(1, sum(2^pred) * scale)); over = #(pred > thresh).
 // stats[0..1] (double sum) and stats[2] (u32 over) must be zero on entry.

@@ -63,7 +63,7 @@ struct nqm_server {
     uint32_t slots_n = 0;
     uint64_t cap_items = 0;   // admission limit of queued items
     uint64_t buf_items = 0;   // physical capacity of each queue buffer
-    nqm::DevBuf q[2], stage[2], counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list,
+    nqm::DevBuf q[2], stage[2], part, counts, head, spill_count, err, target, sols, nodes, exec, outst, slot_list,
         lvl_nodes, cont[2], cont_counts;
     int cur = 0;
     int ci = 0;          // cont[ci] holds the parked pieces
@@ -85,8 +85,8 @@ struct nqm_server {
     uint32_t active = 0;
     uint64_t rounds_total = 0;
     ~nqm_server() {
-        for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &counts, &head, &spill_count, &err, &target,
-                               &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes, &cont[0], &cont[1],
+        for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &part, &counts, &head, &spill_count, &err,
+                               &target, &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes, &cont[0], &cont[1],
                                &cont_counts})
             if (d->ptr) (void)hipFree(d->ptr);
     }
@@ -153,6 +153,7 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t roo
     const uint32_t target = std::min(n - 1, std::max(row, depth));
     uint32_t count = (uint32_t)init.size();
     if ((rc = ensure_dev(ctx, s->stage[0], std::max<size_t>(count, 1) * sizeof(Item))) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, s->part, kExpandPartBytes)) != NQM_OK) return rc;
     if (count)
         NQM_HIP(ctx, hipMemcpyAsync(s->stage[0].ptr, init.data(), count * sizeof(Item), hipMemcpyHostToDevice, st));
     uint32_t* cnt = static_cast<uint32_t*>(s->counts.ptr);
@@ -167,17 +168,18 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t roo
         NQM_HIP(ctx, hipMemcpyAsync(static_cast<uint8_t*>(s->target.ptr) + slot, tgt, 1, hipMemcpyHostToDevice, st));
         if ((rc = ensure_dev(ctx, s->stage[w ^ 1], cap * sizeof(Item))) != NQM_OK) return rc;
         uint32_t* out_slot = cnt + 1 + (w ^ 1);
-        NQM_HIP(ctx, k2_fill_u32(st, out_slot, 1, 0));
         // the level's placements go to scratch counters until the level is kept:
         // lvl_nodes[slot] (weighted, per solve) and lvl_nodes[slots_n] (executed)
         uint64_t* const lvl = static_cast<uint64_t*>(s->lvl_nodes.ptr);
         NQM_HIP(ctx, k2_fill_u64(st, lvl + slot, 1));
         NQM_HIP(ctx, k2_fill_u64(st, lvl + s->slots_n, 1));
-        const uint32_t grid = std::min<uint32_t>(4096u, (count + 255u) / 256u);
-        NQM_HIP(ctx, k2_expand(st, grid, static_cast<Item*>(s->stage[w].ptr), in_slot,
-                               static_cast<Item*>(s->stage[w ^ 1].ptr), out_slot, (uint32_t)cap,
-                               static_cast<uint8_t*>(s->target.ptr), lvl, lvl + s->slots_n,
-                               static_cast<uint32_t*>(s->err.ptr)));
+        const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
+        const Item* const lin = static_cast<Item*>(s->stage[w].ptr);
+        const uint8_t* const tgt_dev = static_cast<uint8_t*>(s->target.ptr);
+        NQM_HIP(ctx, k2_expand_count(st, grid, lin, in_slot, s->part.ptr, out_slot, tgt_dev));
+        NQM_HIP(ctx, k2_expand_write(st, grid, lin, in_slot, s->part.ptr, true,
+                                     static_cast<Item*>(s->stage[w ^ 1].ptr), (uint32_t)cap, tgt_dev, lvl,
+                                     lvl + s->slots_n, static_cast<uint32_t*>(s->err.ptr)));
         uint32_t next = 0;
         uint64_t placed = 0;
         NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
