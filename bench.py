@@ -278,10 +278,39 @@ def cpu_baseline(seconds):
         want = rate * seconds
         stride = max(1, int(stride * nodes / max(want, 1)))
     rate = nodes / dt
+    # one thread, on a sample about a quarter of the time at that rate
+    stride1 = max(1, int(stride * threads * 4))
+    sample1 = np.ascontiguousarray(pool[::stride1])
+    t0 = time.perf_counter()
+    _, nodes1 = oracle.count_pool(N, 8, sample1, mirror=True, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": rate, "unit": "nodes/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}th prefix of the N=18 lock-8 half pool "
                       f"({sample.shape[0]} prefixes, {nodes} nodes, {dt:.1f} s)",
-            "extrapolated_wall_s_n18": HALF_NODES / rate}
+            "extrapolated_wall_s_n18": HALF_NODES / rate,
+            "one_thread": {"value": nodes1 / max(dt1, 1e-9), "unit": "nodes/s",
+                           "sample": f"every {stride1}th prefix ({nodes1} nodes, {dt1:.1f} s)"},
+            "host": host_cpu()}
+
+
+def host_cpu():
+    """CPU model, logical CPUs and SMT state of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "smt_active": smt}
 
 
 def main():

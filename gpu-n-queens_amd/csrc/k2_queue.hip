@@ -117,6 +117,33 @@ __device__ __forceinline__ uint32_t live_children(const Item& it) {
     return nch;
 }
 
+// The candidates of an item whose child has a free cell.
+__device__ __forceinline__ uint32_t live_mask(const Item& it) {
+    const uint32_t mask = (1u << meta_n(it.meta)) - 1u;
+    uint32_t f = it.f, live = 0;
+    while (f) {
+        const uint32_t b = f & (0u - f);
+        f ^= b;
+        live |= (~((it.cols | b) | ((it.ld | b) << 1) | ((it.rd | b) >> 1)) & mask) != 0u ? b : 0u;
+    }
+    return live;
+}
+
+// The child of an item for candidate b.
+__device__ __forceinline__ Item child_item(const Item& it, uint32_t b) {
+    const uint32_t n = meta_n(it.meta);
+    const uint32_t mask = (1u << n) - 1u;
+    Item c;
+    c.cols = it.cols | b;
+    c.ld = ((it.ld | b) << 1) & mask;
+    c.rd = (it.rd | b) >> 1;
+    c.f = ~(c.cols | c.ld | c.rd) & mask;
+    c.solve = it.solve;
+    c.meta = item_meta(n, meta_row(it.meta) + 1u, meta_wshift(it.meta));
+    c.pad0 = c.pad1 = 0;
+    return c;
+}
+
 // Block-wide exclusive scan of one u32 per thread (blockDim = 256); *total
 // receives the block sum. Ends with the scratch free for reuse.
 __device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t* scratch, uint32_t* total) {
@@ -227,6 +254,10 @@ k2_expand_write_kernel(const Item* __restrict__ in, const uint32_t* __restrict__
                        uint64_t* __restrict__ exec_nodes, uint32_t* __restrict__ err,
                        const float* __restrict__ pred, const float* __restrict__ thresh_p) {
     __shared__ uint32_t scratch[4];
+    __shared__ Item s_item[256];
+    __shared__ uint32_t s_live[256];  // live-child mask, 0 for an item copied as is
+    __shared__ uint32_t s_off[256];   // first output of the item within the round
+    __shared__ uint8_t s_map[256u * 31u];  // round output -> item (at most 31 children each)
     const ExpandRule rule{target_row, pred, pred ? *thresh_p : 0.f};
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t c0, c1;
@@ -254,16 +285,15 @@ k2_expand_write_kernel(const Item* __restrict__ in, const uint32_t* __restrict__
         const uint32_t i = b0 + threadIdx.x;
         Item it{};
         bool expand = false;
-        uint32_t nch = 0;
+        uint32_t live = 0, nch = 0;
         if (i < c1) {
             it = in[i];
             expand = rule.expands(it, i);
-            nch = expand ? live_children(it) : 1u;
+            live = expand ? live_mask(it) : 0u;
+            nch = expand ? (uint32_t)__popc(live) : 1u;
         }
         uint32_t step_total = 0;
-        uint32_t w = o + block_scan256(nch, scratch, &step_total);
-        o += step_total;
-        if (i >= c1) continue;
+        const uint32_t wl = block_scan256(nch, scratch, &step_total);
         if (expand && own_nodes) {
             const uint32_t placed = (uint32_t)__popc(it.f);
             if (it.solve != acc_solve && acc_nodes) {
@@ -274,28 +304,25 @@ k2_expand_write_kernel(const Item* __restrict__ in, const uint32_t* __restrict__
             acc_nodes += (uint64_t)placed << meta_wshift(it.meta);
             acc_exec += placed;
         }
-        if (!fits) continue;
-        if (!expand) {
-            out[w] = it;
-            continue;
+        if (fits) {  // block-uniform: stage the round in LDS, then thread k writes outputs k, k+256, ...
+            s_item[threadIdx.x] = it;
+            s_live[threadIdx.x] = live;
+            s_off[threadIdx.x] = wl;
+            for (uint32_t c = 0; c < nch; ++c) s_map[wl + c] = (uint8_t)threadIdx.x;
+            __syncthreads();
+            for (uint32_t k = threadIdx.x; k < step_total; k += 256u) {
+                const uint32_t p = s_map[k];
+                uint32_t lm = s_live[p];
+                Item c = s_item[p];
+                if (lm) {  // the (k - off)-th live candidate
+                    for (uint32_t q = k - s_off[p]; q; --q) lm &= lm - 1u;
+                    c = child_item(c, lm & (0u - lm));
+                }
+                out[o + k] = c;
+            }
+            __syncthreads();
         }
-        const uint32_t n = meta_n(it.meta), row = meta_row(it.meta);
-        const uint32_t mask = (1u << n) - 1u;
-        const uint32_t ws = meta_wshift(it.meta);
-        uint32_t f = it.f;
-        while (f) {
-            const uint32_t b = f & (0u - f);
-            f ^= b;
-            Item c;
-            c.cols = it.cols | b;
-            c.ld = ((it.ld | b) << 1) & mask;
-            c.rd = (it.rd | b) >> 1;
-            c.f = ~(c.cols | c.ld | c.rd) & mask;
-            c.solve = it.solve;
-            c.meta = item_meta(n, row + 1, ws);
-            c.pad0 = c.pad1 = 0;
-            if (c.f) out[w++] = c;
-        }
+        o += step_total;
     }
     if (own_nodes) {  // block-uniform
         add_per_solve(solve_nodes, acc_solve, acc_nodes, lane);
@@ -351,7 +378,7 @@ struct DrainArgs {
     uint32_t cont_cap;
     uint64_t budget;  // 0: drain everything
     uint32_t items_first;
-    uint32_t pad;
+    uint32_t nseg;  // queue segments, each with its own head (head[s * kHeadStride])
     const Cont* cont_in;
     const uint32_t* cont_in_count;
     uint32_t* cont_head;
@@ -525,9 +552,17 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     uint64_t acc_sols = 0, acc_nodes = 0, exec_nodes = 0;
 
     // wave-uniform queue state: items [lq, lq_end) are taken from the queue
-    // and not yet started; rem is what the queue held after the last grab
+    // and not yet started; rem is what the current segment held after the
+    // last grab. The queue is cut into nseg segments with a head each; a
+    // wave starts on the segment of its workgroup (workgroups go round-robin
+    // to the XCDs) and moves on when it finds one used up, so the head
+    // atomics of the chip spread over nseg addresses.
+    const uint32_t nseg = a.nseg;
+    uint32_t seg = blockIdx.x % nseg, seg_done = 0;
+    auto seg_lo = [&](uint32_t sg) { return (uint32_t)((uint64_t)count * sg / nseg); };
+    const uint32_t seg_waves = max(1u, waves / nseg);
     bool queue_open = true;
-    uint32_t lq = 0, lq_end = 0, rem = count;
+    uint32_t lq = 0, lq_end = 0, rem = seg_lo(seg + 1) - seg_lo(seg);
     // parked pieces of earlier rounds (taken after the queue's items)
     const uint32_t cont_count = a.cont_in ? *a.cont_in_count : 0u;
     bool cont_open = cont_count != 0;
@@ -642,13 +677,19 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     // guided self-scheduling: a share of what is left, at
                     // least what the idle lanes need
                     const uint32_t need = nidle - take_local;
-                    uint32_t want = rem / (2u * waves);
+                    uint32_t want = rem / (2u * seg_waves);
                     want = want < need ? need : (want > kGrabMax ? kGrabMax : want);
-                    if (lane == 0) g = atomicAdd(a.head, want);
+                    const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
+                    if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, want);
                     g = __shfl(g, 0, 64);
-                    got = g >= count ? 0u : min(want, count - g);
-                    if (got < want) queue_open = false;
-                    rem = g >= count ? 0u : count - g - got;
+                    got = g >= len ? 0u : min(want, len - g);
+                    rem = g >= len ? 0u : len - g - got;
+                    g += lo;
+                    if (got < want) {  // this segment is used up: the next one
+                        if (++seg_done == nseg) queue_open = false;
+                        seg = seg + 1u == nseg ? 0u : seg + 1u;
+                        rem = seg_lo(seg + 1) - seg_lo(seg);
+                    }
                 }
                 const uint32_t r = lane_rank(idle);
                 uint32_t idx = 0xffffffffu;
@@ -962,7 +1003,7 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.cont_out_count = p.cont_out_count;
     a.cont_cap = p.cont_cap;
     a.items_first = p.items_first;
-    a.pad = 0;
+    a.nseg = p.nseg ? p.nseg : 1u;
     const size_t lds = k2_lds_bytes(p.stack_rows, threads);
     if (threads == 64)
         hipLaunchKernelGGL(k2_drain_kernel<64>, dim3(blocks), dim3(64), lds, s, a);

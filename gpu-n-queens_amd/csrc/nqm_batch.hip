@@ -212,8 +212,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     BatchWork& b = ctx->batch;
     const uint32_t count0 = (uint32_t)h.items.size();
     if ((rc = ensure_dev(ctx, b.init, std::max<size_t>(count0, 1) * sizeof(Item))) != NQM_OK) return rc;
-    for (DevBuf* d : {&b.head, &b.spill_count})
-        if ((rc = ensure_dev(ctx, *d, 64)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.head, kHeadSegs * kHeadStride * 4)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, b.spill_count, 64)) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.counts, 32 * 4)) != NQM_OK) return rc;  // 32 count slots
     if ((rc = ensure_dev(ctx, b.target, std::max<size_t>(ns, 1))) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, b.res, 16 + (size_t)ns * 16)) != NQM_OK) return rc;
@@ -422,6 +422,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     p.donate = (t.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
     p.budget = t.budget;
     p.head = d_head;
+    p.nseg = t.budget ? 1u : kHeadSegs;  // restart rounds read one head
     p.spill_count = d_spill_count;
     p.item_count = d_qcount;
 
@@ -447,7 +448,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = k2_fill_u32(st, d_head, 1, 0);
+        if (e == hipSuccess) e = k2_fill_u32(st, d_head, kHeadSegs * kHeadStride, 0);
         if (e == hipSuccess) e = k2_drain(st, blocks, threads, p);
         hipError_t e2 = hipStreamEndCapture(st, &graph);
         if (e != hipSuccess || e2 != hipSuccess) {
@@ -491,7 +492,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             d_cc = static_cast<uint32_t*>(b.cont_counts.ptr);
         }
         for (;;) {
-            NQM_HIP(ctx, k2_fill_u32(st, d_head, 1, 0));
+            NQM_HIP(ctx, k2_fill_u32(st, d_head, kHeadSegs * kHeadStride, 0));
             NQM_HIP(ctx, k2_fill_u32(st, d_qcount, 1, qn));
             if (rounds_in_place) {
                 if (qb < 0 || qstart + qn + spill_room > b.q[qb].cap / sizeof(Item)) {

@@ -123,7 +123,14 @@ struct K2DrainParams {
     // 1: refill from the queue before resuming parked pieces (serving: new
     // requests first); 0: parked pieces first
     uint32_t items_first = 0;
+    // Queue segments (1 or kHeadSegs), each with its own head at
+    // head[s * kHeadStride]; items_first, the restart rounds and the
+    // continuations need one segment (the head tells where the unstarted
+    // items begin).
+    uint32_t nseg = 1;
 };
+constexpr uint32_t kHeadSegs = 8;     // one per XCD
+constexpr uint32_t kHeadStride = 64;  // u32: one head per 256 bytes
 hipError_t k2_fill_u32(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 hipError_t k2_fill_u64(hipStream_t s, uint64_t* p, uint32_t n);
 // One level of children in two steps, with no same-address atomics (each
@@ -155,7 +162,7 @@ hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pr
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
 // Queue items one drain wave takes per atomic on the queue head (at most);
 // a budget round can hand back up to this many unstarted items per wave.
-constexpr uint32_t kGrabItems = 256;
+constexpr uint32_t kGrabItems = 128;
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
 
