@@ -384,6 +384,7 @@ struct DrainArgs {
     uint32_t* cont_head;
     Cont* cont_out;
     uint32_t* cont_out_count;
+    uint64_t* wave_part;  // nullable: per-wave totals {solve, sols, nodes, exec}
 };
 
 // Lanes that take a piece of another solve (sw) hand over what they counted
@@ -803,11 +804,67 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         pass += ran;
     }
 
-    add_per_solve(a.solve_sols, acc_solve, acc_sols, lane);
-    add_per_solve(a.solve_nodes, acc_solve, acc_nodes, lane);
     exec_nodes = wave_sum64(exec_nodes);
-    if (lane == 0 && exec_nodes)
-        atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec_nodes);
+    if (a.wave_part) {
+        // Store the wave's totals; k2_drain_commit adds them up after the
+        // launch. Thousands of waves ending together would otherwise queue
+        // on the same three counters.
+        const uint64_t act = __ballot((acc_sols | acc_nodes) != 0);
+        const uint32_t first = act ? (uint32_t)__ffsll((long long)act) - 1u : 0u;
+        const uint32_t s0 = __shfl(acc_solve, first, 64);
+        const bool one = __ballot((acc_sols | acc_nodes) != 0 && acc_solve != s0) == 0;
+        uint64_t ws_sols = 0, ws_nodes = 0;
+        if (one) {
+            ws_sols = wave_sum64(acc_sols);
+            ws_nodes = wave_sum64(acc_nodes);
+        } else {  // several solves (rare): add them directly
+            add_per_solve(a.solve_sols, acc_solve, acc_sols, lane);
+            add_per_solve(a.solve_nodes, acc_solve, acc_nodes, lane);
+        }
+        if (lane == 0) {
+            uint64_t* const r = a.wave_part + 4u * (blockIdx.x * (T / 64u) + wave);
+            r[0] = s0;
+            r[1] = ws_sols;
+            r[2] = ws_nodes;
+            r[3] = exec_nodes;
+        }
+    } else {
+        add_per_solve(a.solve_sols, acc_solve, acc_sols, lane);
+        add_per_solve(a.solve_nodes, acc_solve, acc_nodes, lane);
+        if (lane == 0 && exec_nodes)
+            atomicAdd((unsigned long long*)a.exec_nodes, (unsigned long long)exec_nodes);
+    }
+}
+
+// One block: the per-wave totals of a drain launch into the per-solve
+// counters. Neighbouring waves mostly share a solve, so each thread sums a
+// run of waves per solve and the waves add those up (a few atomics).
+__global__ void __launch_bounds__(256)
+k2_drain_commit_kernel(const uint64_t* __restrict__ part, uint32_t nwaves, uint64_t* __restrict__ solve_sols,
+                       uint64_t* __restrict__ solve_nodes, uint64_t* __restrict__ exec_nodes) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t per = (nwaves + 255u) / 256u;
+    const uint32_t j0 = min(nwaves, threadIdx.x * per), j1 = min(nwaves, j0 + per);
+    uint32_t acc_solve = 0;
+    uint64_t acc_sols = 0, acc_nodes = 0, acc_exec = 0;
+    for (uint32_t j = j0; j < j1; ++j) {
+        const uint32_t s = (uint32_t)part[4u * j];
+        const uint64_t ps = part[4u * j + 1u], pn = part[4u * j + 2u];
+        acc_exec += part[4u * j + 3u];
+        if ((ps | pn) == 0) continue;
+        if (s != acc_solve && (acc_sols | acc_nodes) != 0) {
+            if (acc_sols) atomicAdd((unsigned long long*)&solve_sols[acc_solve], (unsigned long long)acc_sols);
+            if (acc_nodes) atomicAdd((unsigned long long*)&solve_nodes[acc_solve], (unsigned long long)acc_nodes);
+            acc_sols = acc_nodes = 0;
+        }
+        acc_solve = s;
+        acc_sols += ps;
+        acc_nodes += pn;
+    }
+    add_per_solve(solve_sols, acc_solve, acc_sols, lane);
+    add_per_solve(solve_nodes, acc_solve, acc_nodes, lane);
+    const uint64_t wexec = wave_sum64(acc_exec);
+    if (lane == 0 && wexec) atomicAdd((unsigned long long*)exec_nodes, (unsigned long long)wexec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1004,6 +1061,7 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.cont_cap = p.cont_cap;
     a.items_first = p.items_first;
     a.nseg = p.nseg ? p.nseg : 1u;
+    a.wave_part = p.wave_part;
     const size_t lds = k2_lds_bytes(p.stack_rows, threads);
     if (threads == 64)
         hipLaunchKernelGGL(k2_drain_kernel<64>, dim3(blocks), dim3(64), lds, s, a);
@@ -1013,6 +1071,9 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
         hipLaunchKernelGGL(k2_drain_kernel<256>, dim3(blocks), dim3(256), lds, s, a);
     else
         return hipErrorInvalidValue;
+    if (p.wave_part)
+        hipLaunchKernelGGL(k2_drain_commit_kernel, dim3(1), dim3(256), 0, s, p.wave_part,
+                           blocks * (threads / 64u), p.solve_sols, p.solve_nodes, p.exec_nodes);
     return hipGetLastError();
 }
 

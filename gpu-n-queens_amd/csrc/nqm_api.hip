@@ -196,7 +196,8 @@ int nqm_create(int device, nqm_ctx** out) {
     ctx->lds_per_cu = (uint32_t)ctx->prop.maxSharedMemoryPerMultiProcessor;
     if (ctx->lds_per_block > 65536) ctx->lds_per_block = 65536;  // static launch limit
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
+        hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
+        hipEventCreate(&ctx->ev_x0) != hipSuccess || hipEventCreate(&ctx->ev_x1) != hipSuccess) {
         nqm_destroy(ctx);
         return NQM_EDEVICE;
     }
@@ -214,11 +215,14 @@ void nqm_destroy(nqm_ctx* ctx) {
     free_dev(ctx->err);
     nqm::BatchWork& w = ctx->batch;
     for (nqm::DevBuf* d : {&w.init, &w.q[0], &w.q[1], &w.counts, &w.head, &w.spill_count, &w.target, &w.res,
-                           &w.part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred, &w.hist, &w.stat})
+                           &w.part, &w.wave_part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred,
+                           &w.hist, &w.stat})
         free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_x0) (void)hipEventDestroy(ctx->ev_x0);
+    if (ctx->ev_x1) (void)hipEventDestroy(ctx->ev_x1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -105,7 +105,10 @@ int build_host_items(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, bool mi
 
 // Expand every item below its solve's target row by one row, level by level,
 // while the items stay under max_items (k2_expand on the host).
-constexpr size_t kHostItems = 1u << 14;
+// Items the host expands to before the device takes over: a single solve
+// stops early (each device level costs a few launches), a mixed batch goes
+// further (its first levels are many small solves).
+constexpr size_t kHostItemsOne = 1u << 10, kHostItemsMany = 1u << 14;
 void host_expand(const std::vector<uint8_t>& target, size_t max_items, HostInit* h) {
     const uint32_t ns = (uint32_t)h->row.size();
     for (;;) {
@@ -207,7 +210,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
 
     // The first levels are tiny: place them on the host (same rule and node
     // accounting as k2_expand) and save a device round trip per level.
-    host_expand(target, kHostItems, &h);
+    host_expand(target, ns == 1 ? kHostItemsOne : kHostItemsMany, &h);
 
     BatchWork& b = ctx->batch;
     const uint32_t count0 = (uint32_t)h.items.size();
@@ -252,36 +255,56 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     NQM_HIP(ctx, enqueue_clear());
 
     // Level L reads level_in(L) (count in slot L) and writes level_out(L)
-    // (count in slot L+1). Each level is counted first, so its buffer is
-    // allocated exactly. Automatic depth keeps expanding while the queue is
+    // (count in slot L+1). Automatic depth keeps expanding while the queue is
     // too short to fill the chip, or while the deepest solve would need more
     // than kRowsGoal LDS stack rows (fewer rows, more resident waves), as long
     // as a level stays under kAutoMaxItems. With a model the expansion stops
     // at want_items: the learned stage splits only what it predicts large.
+    //
+    // Every level is counted on the device before it is written. For a single
+    // solve the host waits for that count only when it decides something: a
+    // level that the rows goal (or an explicit depth) asks for regardless of
+    // its size, and whose upper bound fits kAutoMaxItems, is sized by the
+    // bound (an item at row r has at most n - r children) and enqueued
+    // without a round trip. (Mixed batches measured slower that way: their
+    // loose bounds inflate the next level's grid.)
     constexpr uint32_t kRowsGoal = 9;
     constexpr uint64_t kAutoMaxItems = 1ull << 27;
     std::vector<uint32_t> caps, ins;
     std::vector<uint8_t> final_row(h.row);
-    uint32_t count = count0;
+    uint32_t count = count0;  // size of the current level (an upper bound unless exact)
+    bool exact = true;
     float expand_ms = 0.f;
-    NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_x0, st));
     for (uint32_t L = 0; L < 11; ++L) {
         bool any = false;
-        uint32_t rows_left = 0;
+        uint32_t rows_left = 0, grow = 1;
         for (uint32_t s = 0; s < ns; ++s)
             if (final_row[s] < target[s]) {
                 any = true;
                 rows_left = std::max(rows_left, solves[s].n - 1u - final_row[s]);
+                grow = std::max(grow, solves[s].n - final_row[s]);
             }
         if (!any || count == 0) break;
-        if (auto_depth && (uint64_t)count >= want_items && (t.model || rows_left <= kRowsGoal)) break;
+        const uint64_t bound = (uint64_t)count * grow;
+        const bool blind =
+            ns == 1 && bound <= kAutoMaxItems && (!auto_depth || (!t.model && rows_left > kRowsGoal));
+        if (!blind && !exact) {  // the decision below needs the level's true size
+            NQM_HIP(ctx, hipMemcpyAsync(&count, d_cnt + L, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            exact = true;
+            if (count == 0) break;
+        }
+        if (auto_depth && !blind && (uint64_t)count >= want_items && (t.model || rows_left <= kRowsGoal)) break;
         const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
         NQM_HIP(ctx, k2_expand_count(st, grid, level_in(L), d_cnt + L, d_part, d_cnt + L + 1, d_target));
-        uint32_t next = 0;
-        NQM_HIP(ctx, hipMemcpyAsync(&next, d_cnt + L + 1, 4, hipMemcpyDeviceToHost, st));
-        NQM_HIP(ctx, hipStreamSynchronize(st));
-        if (auto_depth && next > kAutoMaxItems && (uint64_t)count >= want_items) break;
-        if (next > 0x7fffffffu) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
+        uint32_t next = (uint32_t)bound;  // blind: capacity
+        if (!blind) {
+            NQM_HIP(ctx, hipMemcpyAsync(&next, d_cnt + L + 1, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            if (auto_depth && next > kAutoMaxItems && (uint64_t)count >= want_items) break;
+            if (next > 0x7fffffffu) return fail(ctx, NQM_ERANGE, "expansion exceeds 2^31 items");
+        }
         if ((rc = ensure_dev(ctx, b.q[L & 1], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK) return rc;
         NQM_HIP(ctx, k2_expand_write(st, grid, level_in(L), d_cnt + L, d_part, !t.model, level_out(L), next,
                                      d_target, d_nodes, d_exec, d_err));
@@ -290,10 +313,17 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         for (uint32_t s = 0; s < ns; ++s)
             if (final_row[s] < target[s]) final_row[s]++;
         count = next;
+        exact = !blind;
     }
-    NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-    NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
-    NQM_HIP(ctx, hipEventElapsedTime(&expand_ms, ctx->ev_start, ctx->ev_stop));
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_x1, st));
+    // A single-round drain without a model reads its queue size on the
+    // device: nothing below needs the host to wait for the expansion.
+    const bool host_count = !exact && (t.model || t.budget || use_graph);
+    if (host_count) {
+        NQM_HIP(ctx, hipMemcpyAsync(&count, d_cnt + caps.size(), 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        exact = true;
+    }
     const uint32_t levels = (uint32_t)caps.size();
     Item* final_q = level_in(levels);
     int spare = levels == 0 ? 0 : (int)(levels & 1);  // q[] not holding final_q
@@ -423,6 +453,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     p.budget = t.budget;
     p.head = d_head;
     p.nseg = t.budget ? 1u : kHeadSegs;  // restart rounds read one head
+    if ((rc = ensure_dev(ctx, b.wave_part, (size_t)blocks * (threads / 64u) * 32u)) != NQM_OK) return rc;
+    p.wave_part = static_cast<uint64_t*>(b.wave_part.ptr);
     p.spill_count = d_spill_count;
     p.item_count = d_qcount;
 
@@ -493,7 +525,10 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         }
         for (;;) {
             NQM_HIP(ctx, k2_fill_u32(st, d_head, kHeadSegs * kHeadStride, 0));
-            NQM_HIP(ctx, k2_fill_u32(st, d_qcount, 1, qn));
+            if (exact)
+                NQM_HIP(ctx, k2_fill_u32(st, d_qcount, 1, qn));
+            else  // the expansion's last count, left on the device
+                NQM_HIP(ctx, hipMemcpyAsync(d_qcount, final_slot, 4, hipMemcpyDeviceToDevice, st));
             if (rounds_in_place) {
                 if (qb < 0 || qstart + qn + spill_room > b.q[qb].cap / sizeof(Item)) {
                     const int to = qb < 0 ? spare : qb ^ 1;
@@ -560,6 +595,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     const uint64_t* const hs = res.data() + 2;
     const uint64_t* const hn = hs + ns;
     NQM_HIP(ctx, hipEventElapsedTime(&run_ms, ctx->ev_start, ctx->ev_stop));
+    NQM_HIP(ctx, hipEventElapsedTime(&expand_ms, ctx->ev_x0, ctx->ev_x1));
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (err & 16u) return fail(ctx, NQM_EDEVICE, "drain LDS does not start at address 0");

@@ -31,7 +31,7 @@ namespace nqm {
 struct BatchWork {
     // res: u64 executed nodes, u32 error bits, u32 pad, then per solve u64
     // solutions and u64 nodes (one copy brings every result back)
-    DevBuf init, q[2], counts, head, spill_count, target, res, part;
+    DevBuf init, q[2], counts, head, spill_count, target, res, part, wave_part;
     DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
     DevBuf feats, pred, hist, stat;  // learned splitting
 };
@@ -41,6 +41,7 @@ struct nqm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_x0 = nullptr, ev_x1 = nullptr;  // the batch expansion
     hipDeviceProp_t prop{};
     uint32_t lds_per_block = 0;
     uint32_t lds_per_cu = 0;
@@ -128,6 +129,10 @@ struct K2DrainParams {
     // continuations need one segment (the head tells where the unstarted
     // items begin).
     uint32_t nseg = 1;
+    // Per-wave totals, 32 bytes per wave of the launch (nullable): each wave
+    // stores its counts there and one block adds them to the per-solve
+    // counters after the drain, instead of every wave adding them at its end.
+    uint64_t* wave_part = nullptr;
 };
 constexpr uint32_t kHeadSegs = 8;     // one per XCD
 constexpr uint32_t kHeadStride = 64;  // u32: one head per 256 bytes
