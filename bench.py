@@ -2,14 +2,18 @@
 """Benchmark: exact N-Queens count at N=18 on one MI355X.
 
 The metric is wall-seconds to the exact count at N=18 plus search nodes/s.
-A step is one complete N=18 count (666,090,624 solutions) by one of two paths:
+A step is one complete N=18 count (666,090,624 solutions) by one of these paths:
 
   plain      -- K1, the static one-prefix-per-lane kernel, on the reference's
                 default pool (the lock row its generator picks for 8e7
                 requested states: lock 8, 37,956,796 mirror-half prefixes),
                 generated on the device and resident in HBM before timing;
   persistent -- K2, expand + persistent queue drain with in-wave donation,
-                from the empty board (9 row-1 items) on the device;
+                from the empty board on the device;
+  persistent_graph -- the same work as one HIP graph: the first (warm-up)
+                call plans the expansion levels and captures them with the
+                drain, every timed step launches the graph (clears, every
+                expansion level, the drain) and copies the counts back;
   learned    -- K2 with the subtree-size MLP: items predicted large are split
                 deeper and the queue runs largest-predicted first. The model
                 is trained first (untimed) on the solver's own node counts of
@@ -23,7 +27,7 @@ time against the INT32 VALU peak (CUs x 64 lanes x engine clock) at 12
 integer ops per node. The cpu_baseline object times the CPU oracle (a port of
 the reference's counting loop) on a bounded sample of the same pool.
 
-  python bench.py [--steps K] [--warmup W] [--variant all|plain|persistent|learned]
+  python bench.py [--steps K] [--warmup W] [--variant all|plain|persistent|persistent_graph|learned]
 
 Other workloads (one JSON line each, not the headline metric):
   --workload batch    512 seeded mixed solves (N 8..17, 0-4 pre-placed queens)
@@ -55,7 +59,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="all", choices=["all", "plain", "persistent", "learned"])
+    ap.add_argument("--variant", default="all",
+                    choices=["all", "plain", "persistent", "persistent_graph", "learned"])
     ap.add_argument("--epochs", type=int, default=8, help="MLP training epochs (learned variant)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -352,6 +357,13 @@ def main():
         results["plain"] = run_plain(ctx, nqm, args, lock)
     if args.variant in ("all", "persistent"):
         results["persistent"] = run_persistent(ctx, nqm, args, None)
+    if args.variant in ("all", "persistent_graph"):
+        # the same path as one HIP graph: the first (warm-up) call plans the
+        # levels and captures them with the drain; every timed step launches
+        # that graph again (clears, expansion, drain, result copy)
+        r = run_persistent(ctx, nqm, args, nqm.make_tune(flags=nqm.TUNE_GRAPH))
+        r["path"] = "K2 persistent queue, replayed as one HIP graph"
+        results["persistent_graph"] = r
     if args.variant in ("all", "learned"):
         results["learned"] = run_learned(ctx, nqm, args)
 
@@ -372,8 +384,11 @@ def main():
         }
     best = min(variants, key=lambda k: variants[k]["ms_per_step"])
     b = variants[best]
-    # the search kernel alone: K1's count launch, or K2's drain launch(es)
-    achieved = OPS_PER_NODE * b["kernel_nodes"] / (b["search_ms"] / 1e3)
+    # the search kernel alone: K1's count launch, or K2's drain launch(es).
+    # A graph is timed as a whole; its drain (the same kernel on the same
+    # work) is timed by the plain persistent variant.
+    rb = variants.get(best.replace("_graph", ""), b)
+    achieved = OPS_PER_NODE * rb["kernel_nodes"] / (rb["search_ms"] / 1e3)
     line = {
         "metric": METRIC,
         "value": b["nodes_per_s"],
@@ -393,13 +408,13 @@ def main():
         "wall_s_to_count": b["wall_s_to_count"],
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_ops / 1e12,
                      "unit": "TOP/s", "frac": achieved / peak_ops,
-                     "traffic": traffic_from_profiles().get(best),
-                     "ops_per_node": OPS_PER_NODE, "kernel_nodes": b["kernel_nodes"],
-                     "kernel": b["kernel"], "kernel_ms": b["search_ms"]},
+                     "traffic": traffic_from_profiles().get(best.replace("_graph", "")),
+                     "ops_per_node": OPS_PER_NODE, "kernel_nodes": rb["kernel_nodes"],
+                     "kernel": rb["kernel"], "kernel_ms": rb["search_ms"]},
         "variants": variants,
         "device": info,
     }
-    for v in ("persistent", "learned"):
+    for v in ("persistent", "persistent_graph", "learned"):
         if "plain" in variants and v in variants:
             line[f"{v}_vs_plain"] = variants["plain"]["ms_per_step"] / variants[v]["ms_per_step"]
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

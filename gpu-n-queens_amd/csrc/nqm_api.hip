@@ -214,6 +214,7 @@ void nqm_destroy(nqm_ctx* ctx) {
     free_dev(ctx->part_nodes);
     free_dev(ctx->err);
     nqm::BatchWork& w = ctx->batch;
+    nqm::drop_plan(w.plan);
     for (nqm::DevBuf* d : {&w.init, &w.q[0], &w.q[1], &w.counts, &w.head, &w.spill_count, &w.target, &w.res,
                            &w.part, &w.wave_part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred,
                            &w.hist, &w.stat})

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/nqm.h"
 
@@ -28,13 +29,27 @@ struct HostBuf {
 namespace nqm {
 // Device buffers of the batch solver, kept on the context so that repeated
 // calls reuse them (they only grow; nqm_destroy frees them).
+// The last graph-mode batch, kept instantiated: a call that repeats its
+// solves and tuning launches it again. Any other batch call on the context
+// drops it (the graph reads the context's buffers and their contents).
+struct GraphPlan {
+    bool valid = false;
+    std::vector<uint8_t> key;  // the solves and the tuning it was built for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> host_sols, host_nodes;  // placed on the host when it was built
+    uint64_t host_exec = 0;
+    uint32_t blocks = 0;
+};
 struct BatchWork {
     // res: u64 executed nodes, u32 error bits, u32 pad, then per solve u64
     // solutions and u64 nodes (one copy brings every result back)
     DevBuf init, q[2], counts, head, spill_count, target, res, part, wave_part;
     DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
     DevBuf feats, pred, hist, stat;  // learned splitting
+    GraphPlan plan;
 };
+void drop_plan(GraphPlan& plan);
 }  // namespace nqm
 
 struct nqm_ctx {

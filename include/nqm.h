@@ -188,7 +188,9 @@ typedef struct {
 
 #define NQM_TUNE_NO_MIRROR 0x1u  /* do not halve empty boards by mirror symmetry */
 #define NQM_TUNE_NO_DONATE 0x2u  /* no in-wave work donation (queue refill only) */
-#define NQM_TUNE_GRAPH 0x4u      /* run expand + drain as one captured HIP graph */
+#define NQM_TUNE_GRAPH 0x4u      /* run expand + drain as one captured HIP graph; a
+                                    call repeating the previous call's solves and
+                                    tuning launches that graph again */
 #define NQM_TUNE_NO_ORDER 0x8u   /* with a model: split, but keep the queue order */
 
 struct nqm_mlp;
@@ -204,8 +206,7 @@ typedef struct {
     uint32_t flags;        /* NQM_TUNE_* */
     uint32_t stack_rows;   /* LDS stack rows per lane (the expansion then goes deep
                               enough that no piece needs more); 0 = automatic: what
-                              the deepest piece needs, one less from 8 rows on (a row
-                              that finds its stack full is queued as an item) */
+                              the deepest piece needs */
     /* Learned splitting: with a model, the expansion stops early, every item
      * whose predicted log2(1 + nodes) exceeds split_log2 is split one row
      * deeper, up to split_rounds times (0 = 1), the queue is ordered largest

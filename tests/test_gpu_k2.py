@@ -91,6 +91,27 @@ def test_graph_replay_is_stable(ctx, known_totals):
         assert int(c[1]) == oracle.count_prefix(14, [1, 3])[0]
 
 
+def test_graph_plan_replays_and_is_dropped(ctx, known_totals):
+    """A graph-mode call that repeats the previous one launches the kept
+    graph again; any other call (another batch, or a plain one) drops it.
+    Every replay is checked against the eager result, with eager work and
+    synchronizations between the replays."""
+    a = [(15, ()), (14, (1, 3)), (12, ())]
+    b = [(13, ()), (11, (0,))]
+    eager = {}
+    for name, solves in (("a", a), ("b", b)):
+        c, nd, _ = ctx.solve_batch(solves)
+        eager[name] = (list(c), list(nd))
+    assert eager["a"][0][0] == known_totals[15] and eager["b"][0][0] == known_totals[13]
+    for name in ("a", "a", "a", "b", "a", "b", "b", "plain-a", "a", "a"):
+        solves = a if name.endswith("a") else b
+        flags = 0 if name.startswith("plain") else nqm.TUNE_GRAPH
+        c, nd, st = ctx.solve_batch(solves, flags=flags)
+        key = "a" if name.endswith("a") else "b"
+        assert (list(c), list(nd)) == eager[key], name
+        assert st.total == sum(eager[key][0])
+
+
 def test_full_and_dead_prefixes(ctx):
     full = (0, 4, 7, 5, 2, 6, 1, 3)  # an 8-queens solution
     dead = (0, 2, 4, 1, 3)           # row 5 of N=8 has no free cell? check with oracle
