@@ -391,7 +391,6 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     uint32_t split_levels = 0;
     if (t.model && count) {
         const uint32_t rounds = t.split_rounds ? t.split_rounds : 1u;
-        const double lanes = (double)ctx->prop.multiProcessorCount * 16.0 * 64.0;
         DevBuf& feats = b.feats;
         DevBuf& pred = b.pred;
         DevBuf& hist = b.hist;
@@ -414,10 +413,14 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         for (uint32_t r = 0; r <= rounds; ++r) {
             if ((rc = predict(final_q, final_slot, count)) != NQM_OK) return rc;
             if (r == rounds) break;
-            // split what exceeds 1/8 of a lane's fair share (or the fixed threshold)
+            // Split the heavy tail: items predicted above 8x the level's mean
+            // (or the fixed threshold). The largest-first order handles the
+            // rest, and the rows-goal expansion that follows divides every
+            // item ~10^2-fold anyway; splitting nearly everything (a lane's
+            // fair share as the bar) only added a full level of copies.
             NQM_HIP(ctx, k2_fill_u32(st, reinterpret_cast<uint32_t*>(d_psum), 4, 0));
             NQM_HIP(ctx, k2_split_threshold(st, static_cast<const float*>(pred.ptr), final_slot, count,
-                                            t.split_log2, 1.0 / (8.0 * lanes), d_psum, d_over, d_thresh));
+                                            t.split_log2, 8.0 / std::max(count, 1u), d_psum, d_over, d_thresh));
             uint32_t over = 0;
             NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));

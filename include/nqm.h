@@ -212,7 +212,7 @@ typedef struct {
      * deeper, up to split_rounds times (0 = 1), the queue is ordered largest
      * predicted subtree first, and (automatic depth) the expansion then
      * continues to the stack-depth goal keeping that order. split_log2 = 0
-     * picks log2(sum / (8 x lanes)). */
+     * picks log2(8 x the mean predicted size of the level). */
     struct nqm_mlp* model;
     float split_log2;
     uint32_t split_rounds;
