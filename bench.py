@@ -17,7 +17,10 @@ A step is one complete N=18 count (666,090,624 solutions) by one of these paths:
   learned    -- K2 with the subtree-size MLP: items predicted large are split
                 deeper and the queue runs largest-predicted first. The model
                 is trained first (untimed) on the solver's own node counts of
-                every depth-4..6 prefix of N=13..16, by the device kernels.
+                every depth-4..6 prefix of N=13..16, by the device kernels;
+  learned_graph -- the same with the learned stage (features, predictions,
+                split, ordering) captured in the graph with the expansion and
+                the drain; every timed step launches that graph.
 
 `value` is executed search nodes per second of the best path, over the timed
 steps (host wall clock, synchronized). Nodes are queen placements; the
@@ -27,7 +30,8 @@ time against the INT32 VALU peak (CUs x 64 lanes x engine clock) at 12
 integer ops per node. The cpu_baseline object times the CPU oracle (a port of
 the reference's counting loop) on a bounded sample of the same pool.
 
-  python bench.py [--steps K] [--warmup W] [--variant all|plain|persistent|persistent_graph|learned]
+  python bench.py [--steps K] [--warmup W]
+                  [--variant all|plain|persistent|persistent_graph|learned|learned_graph]
 
 Other workloads (one JSON line each, not the headline metric):
   --workload batch    512 seeded mixed solves (N 8..17, 0-4 pre-placed queens)
@@ -60,7 +64,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--variant", default="all",
-                    choices=["all", "plain", "persistent", "persistent_graph", "learned"])
+                    choices=["all", "plain", "persistent", "persistent_graph", "learned", "learned_graph"])
     ap.add_argument("--epochs", type=int, default=8, help="MLP training epochs (learned variant)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,21 +120,33 @@ def run_persistent(ctx, nqm, args, tune):
             "kernel": "k2_drain_kernel"}
 
 
-def run_learned(ctx, nqm, args):
+def run_learned(ctx, nqm, args, names):
+    """Train the model once, then time the learned path eagerly ("learned")
+    and/or as one replayed graph ("learned_graph")."""
     from nqm import learned
     t0 = time.perf_counter()
     X, Y = learned.build_dataset(ctx)
     model, info = learned.train(ctx, X, Y, epochs=args.epochs)
     info["dataset_and_train_s"] = time.perf_counter() - t0
     info["n18_depth6"] = learned.prediction_error(ctx, model, N, 6)
-    tune = nqm.make_tune(model=model)
+    out = {}
     try:
-        r = run_persistent(ctx, nqm, args, tune)
+        if "learned" in names:
+            r = run_persistent(ctx, nqm, args, nqm.make_tune(model=model))
+            r["path"] = "K2 persistent queue + learned splitting"
+            out["learned"] = r
+        if "learned_graph" in names:
+            # the first (warm-up) call plans and captures expansion, learned
+            # stage and drain; every timed step launches that graph again
+            r = run_persistent(ctx, nqm, args, nqm.make_tune(model=model, flags=nqm.TUNE_GRAPH))
+            r["path"] = "K2 persistent queue + learned splitting, replayed as one HIP graph"
+            out["learned_graph"] = r
     finally:
         model.close()
-    r["path"] = "K2 persistent queue + learned splitting"
-    r["model"] = info
-    return r
+    first = next(iter(out.values()), None)
+    if first is not None:  # the model is described once
+        first["model"] = info
+    return out
 
 
 def traffic_from_profiles():
@@ -364,8 +380,9 @@ def main():
         r = run_persistent(ctx, nqm, args, nqm.make_tune(flags=nqm.TUNE_GRAPH))
         r["path"] = "K2 persistent queue, replayed as one HIP graph"
         results["persistent_graph"] = r
-    if args.variant in ("all", "learned"):
-        results["learned"] = run_learned(ctx, nqm, args)
+    if args.variant in ("all", "learned", "learned_graph"):
+        names = ("learned", "learned_graph") if args.variant == "all" else (args.variant,)
+        results.update(run_learned(ctx, nqm, args, names))
 
     variants = {}
     for name, r in results.items():
@@ -414,7 +431,7 @@ def main():
         "variants": variants,
         "device": info,
     }
-    for v in ("persistent", "persistent_graph", "learned"):
+    for v in ("persistent", "persistent_graph", "learned", "learned_graph"):
         if "plain" in variants and v in variants:
             line[f"{v}_vs_plain"] = variants["plain"]["ms_per_step"] / variants[v]["ms_per_step"]
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

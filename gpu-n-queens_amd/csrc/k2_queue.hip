@@ -918,16 +918,17 @@ constexpr uint32_t kBuckets = 64;
 
 __device__ __forceinline__ uint32_t pred_bucket(float p) {
     // bucket 0 holds the largest predictions (log2 sizes 31.5 and up)
+    // (a NaN prediction goes to bucket 0)
     const float b = (32.0f - p) * 2.0f;
-    return b <= 0.f ? 0u : (b >= (float)(kBuckets - 1) ? kBuckets - 1 : (uint32_t)b);
+    return !(b > 0.f) ? 0u : (b >= (float)(kBuckets - 1) ? kBuckets - 1 : (uint32_t)b);
 }
 
 __global__ void k2_bucket_hist_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count,
-                                      uint32_t* __restrict__ hist) {
+                                      uint32_t max_count, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kBuckets];
     for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    const uint32_t c = *count;
+    const uint32_t c = min(*count, max_count);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
         atomicAdd(&h[pred_bucket(pred[i])], 1u);
     __syncthreads();
@@ -951,11 +952,11 @@ __global__ void k2_bucket_scan_kernel(uint32_t* __restrict__ hist) {
 // inside the range with LDS atomics. Order inside a bucket is arbitrary.
 __global__ void __launch_bounds__(256)
 k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ pred,
-                         const uint32_t* __restrict__ count, uint32_t* __restrict__ cursor,
+                         const uint32_t* __restrict__ count, uint32_t max_count, uint32_t* __restrict__ cursor,
                          Item* __restrict__ out) {
     __shared__ uint32_t h[kBuckets];
     __shared__ uint32_t off[kBuckets];
-    const uint32_t c = *count;
+    const uint32_t c = min(*count, max_count);
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < c; b0 += gridDim.x * blockDim.x) {
         for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
         __syncthreads();
@@ -978,30 +979,48 @@ hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pr
                                   uint32_t max_count, uint32_t* hist, Item* out) {
     const uint32_t blocks = std::max(1u, std::min(2048u, (max_count + 255u) / 256u));
     hipLaunchKernelGGL(k2_fill_u32_kernel, dim3(1), dim3(64), 0, s, hist, kBuckets, 0u);
-    hipLaunchKernelGGL(k2_bucket_hist_kernel, dim3(blocks), dim3(256), 0, s, pred, count, hist);
+    hipLaunchKernelGGL(k2_bucket_hist_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, hist);
     hipLaunchKernelGGL(k2_bucket_scan_kernel, dim3(1), dim3(64), 0, s, hist);
-    hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, hist, out);
+    hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, max_count, hist,
+                       out);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // learned split threshold, computed where the predictions are
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-k2_pred_sum_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, double* __restrict__ sum) {
-    __shared__ double part[4];
-    const uint32_t c = *count;
-    double v = 0.0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
-        v += exp2((double)pred[i]);
+// The sum of 2^pred is formed in a fixed order (per-block partials, then
+// one block adds them), so that the threshold, and with it every size the
+// split decides, is the same on every run over the same items: a captured
+// graph replays the sizes of the run that planned it.
+__device__ __forceinline__ double block_sum256_f64(double v, double* part) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (uint32_t w = 0; w < blockDim.x / 64u; ++w) t += part[w];
-        if (t != 0.0) atomicAdd(sum, t);
-    }
+    double t = 0.0;
+    for (uint32_t w = 0; w < blockDim.x / 64u; ++w) t += part[w];
+    return t;
+}
+
+__global__ void __launch_bounds__(256)
+k2_pred_sum_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, uint32_t max_count,
+                   double* __restrict__ block_sums) {
+    __shared__ double part[4];
+    const uint32_t c = min(*count, max_count);
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
+        v += exp2((double)pred[i]);
+    const double t = block_sum256_f64(v, part);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256)
+k2_pred_total_kernel(const double* __restrict__ block_sums, uint32_t blocks, double* __restrict__ sum) {
+    __shared__ double part[4];
+    double v = 0.0;
+    for (uint32_t i = threadIdx.x; i < blocks; i += blockDim.x) v += block_sums[i];
+    const double t = block_sum256_f64(v, part);
+    if (threadIdx.x == 0) *sum = t;
 }
 
 __device__ __forceinline__ float split_thresh(float fixed, double scale, const double* sum) {
@@ -1009,10 +1028,10 @@ __device__ __forceinline__ float split_thresh(float fixed, double scale, const d
 }
 
 __global__ void __launch_bounds__(256)
-k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, float fixed,
-                    double scale, const double* __restrict__ sum, uint32_t* __restrict__ over,
+k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, uint32_t max_count,
+                    float fixed, double scale, const double* __restrict__ sum, uint32_t* __restrict__ over,
                     float* __restrict__ thresh) {
-    const uint32_t c = *count;
+    const uint32_t c = min(*count, max_count);
     const float t = split_thresh(fixed, scale, sum);
     if (blockIdx.x == 0 && threadIdx.x == 0) *thresh = t;
     uint32_t n = 0;
@@ -1023,12 +1042,15 @@ k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__
 }
 
 hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
-                              float fixed, double scale, double* sum, uint32_t* over, float* thresh) {
-    const uint32_t blocks = std::max(1u, std::min(1024u, (max_count + 255u) / 256u));
-    if (!(fixed > 0.f))
-        hipLaunchKernelGGL(k2_pred_sum_kernel, dim3(blocks), dim3(256), 0, s, pred, count, sum);
-    hipLaunchKernelGGL(k2_pred_over_kernel, dim3(blocks), dim3(256), 0, s, pred, count, fixed, scale, sum, over,
-                       thresh);
+                              float fixed, double scale, double* block_sums, double* sum, uint32_t* over,
+                              float* thresh) {
+    const uint32_t blocks = std::max(1u, std::min(kPredSumBlocks, (max_count + 255u) / 256u));
+    if (!(fixed > 0.f)) {
+        hipLaunchKernelGGL(k2_pred_sum_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, block_sums);
+        hipLaunchKernelGGL(k2_pred_total_kernel, dim3(1), dim3(256), 0, s, block_sums, blocks, sum);
+    }
+    hipLaunchKernelGGL(k2_pred_over_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, fixed, scale,
+                       sum, over, thresh);
     return hipGetLastError();
 }
 

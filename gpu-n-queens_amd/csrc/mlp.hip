@@ -17,6 +17,7 @@
 // operand, so activations never leave registers. The weight-gradient GEMMs
 // reduce over samples and stage their operands through wave-private LDS.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -82,8 +83,8 @@ __global__ void mlp_features_pool_kernel(const uint8_t* __restrict__ pool, uint6
 }
 
 __global__ void mlp_features_items_kernel(const Item* __restrict__ items, const uint32_t* __restrict__ count,
-                                          _Float16* __restrict__ feats) {
-    const uint32_t c = *count;
+                                          uint32_t max_count, _Float16* __restrict__ feats) {
+    const uint32_t c = min(*count, max_count);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x) {
         const Item it = items[i];
         item_features(meta_n(it.meta), meta_row(it.meta), it.cols, it.ld, it.rd, feats + (size_t)i * kIn);
@@ -314,9 +315,19 @@ struct nqm_mlp {
     nqm_ctx* ctx = nullptr;
     nqm::DevBuf w, m, v, g, wh, loss;
     int step = 0;
+    uint64_t version = 0;  // new value from g_mlp_version at every weight change
 };
 
 namespace nqm {
+
+// Weight versions: every change of any model's weights takes the next value,
+// so that a batch plan keyed by (model, version) never outlives the weights
+// it was planned with, even if a new model reuses a freed model's address.
+static std::atomic<uint64_t> g_mlp_version{0};
+
+static void mlp_touch(nqm_mlp* m) { m->version = g_mlp_version.fetch_add(1, std::memory_order_relaxed) + 1; }
+
+uint64_t mlp_version(const nqm_mlp* m) { return m->version; }
 
 static uint64_t mix64(uint64_t& s) {  // splitmix64
     s += 0x9E3779B97F4A7C15ull;
@@ -365,7 +376,7 @@ int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y
 hipError_t mlp_features_items(hipStream_t s, const Item* items, const uint32_t* count, void* feats,
                               uint32_t max_count) {
     const uint32_t blocks = std::max(1u, std::min(4096u, (max_count + 255u) / 256u));
-    hipLaunchKernelGGL(mlp_features_items_kernel, dim3(blocks), dim3(256), 0, s, items, count,
+    hipLaunchKernelGGL(mlp_features_items_kernel, dim3(blocks), dim3(256), 0, s, items, count, max_count,
                        static_cast<_Float16*>(feats));
     return hipGetLastError();
 }
@@ -426,6 +437,7 @@ int nqm_mlp_set_params(nqm_mlp* m, const float* params) {
     if (!m || !params) return NQM_EINVAL;
     nqm_ctx* ctx = m->ctx;
     hipStream_t st = ctx->stream;
+    mlp_touch(m);
     NQM_HIP(ctx, hipMemcpyAsync(m->w.ptr, params, kParams * 4, hipMemcpyHostToDevice, st));
     NQM_HIP(ctx, hipMemsetAsync(m->m.ptr, 0, kParams * 4, st));
     NQM_HIP(ctx, hipMemsetAsync(m->v.ptr, 0, kParams * 4, st));
@@ -489,6 +501,7 @@ int nqm_mlp_train(nqm_mlp* m, const void* d_feats, const float* d_labels, uint32
         return NQM_EINVAL;
     nqm_ctx* ctx = m->ctx;
     hipStream_t st = ctx->stream;
+    mlp_touch(m);
     // seeded split: a permutation; the first (1 - val_frac) part trains
     std::vector<uint32_t> perm(count);
     for (uint32_t i = 0; i < count; ++i) perm[i] = i;

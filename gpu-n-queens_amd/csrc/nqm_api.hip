@@ -217,7 +217,7 @@ void nqm_destroy(nqm_ctx* ctx) {
     nqm::drop_plan(w.plan);
     for (nqm::DevBuf* d : {&w.init, &w.q[0], &w.q[1], &w.counts, &w.head, &w.spill_count, &w.target, &w.res,
                            &w.part, &w.wave_part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred,
-                           &w.hist, &w.stat})
+                           &w.hist, &w.stat, &w.goal})
         free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);

@@ -13,9 +13,10 @@
 //      drain is relaunched over them (continuations first) until nothing is
 //      left (restart);
 //   4. per-solve u64 counters come back in one copy.
-// With NQM_TUNE_GRAPH, steps 2-4 are captured once into a HIP graph (the
-// level sizes learned by a first, uncaptured expansion) and launched as one
-// graph; counters are cleared by fill kernels, never memset nodes.
+// With NQM_TUNE_GRAPH, steps 2-4 (and the learned stage, with a model) are
+// captured once into a HIP graph (the sizes learned by a first, uncaptured
+// run) and launched as one graph; counters are cleared by fill kernels, never
+// memset nodes.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -230,15 +231,15 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 8u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
     const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
-    if ((t.flags & NQM_TUNE_GRAPH) && t.model)
-        return fail(ctx, NQM_EUNSUPPORTED, "learned splitting decides on the host: no graph capture");
 
-    // A graph-mode call that repeats the previous call's solves and tuning
-    // launches the graph built then; any other call drops that graph.
+    // A graph-mode call that repeats the previous call's solves, tuning and
+    // model weights launches the graph built then; any other call drops it.
     BatchWork& b = ctx->batch;
-    std::vector<uint8_t> key(sizeof(nqm_tune) + (size_t)ns * sizeof(nqm_solve));
+    const uint64_t model_version = t.model ? mlp_version(t.model) : 0u;
+    std::vector<uint8_t> key(sizeof(nqm_tune) + sizeof(model_version) + (size_t)ns * sizeof(nqm_solve));
     std::memcpy(key.data(), &t, sizeof(nqm_tune));
-    std::memcpy(key.data() + sizeof(nqm_tune), solves, (size_t)ns * sizeof(nqm_solve));
+    std::memcpy(key.data() + sizeof(nqm_tune), &model_version, sizeof(model_version));
+    std::memcpy(key.data() + sizeof(nqm_tune) + sizeof(model_version), solves, (size_t)ns * sizeof(nqm_solve));
     if (use_graph && b.plan.valid && b.plan.key == key) return replay_plan(ctx, ns, counts, nodes_out, stats, t0);
     drop_plan(b.plan);
 
@@ -387,107 +388,143 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     uint32_t* final_slot = d_cnt + levels;
 
     // ---- learned splitting: split predicted-large items, order largest first
+    //
+    // The stage runs live, the host reading the sizes it decides on, and in
+    // graph mode once more inside the capture with the sizes of the live run.
+    // Features, predictions and the split threshold are formed in a fixed
+    // order, so every size is the same on every run over the same items and
+    // weights (the plan key holds the weights' version).
     float learn_ms = 0.f;
     uint32_t split_levels = 0;
-    if (t.model && count) {
-        const uint32_t rounds = t.split_rounds ? t.split_rounds : 1u;
-        DevBuf& feats = b.feats;
-        DevBuf& pred = b.pred;
-        DevBuf& hist = b.hist;
-        DevBuf& stat = b.stat;
-        if ((rc = ensure_dev(ctx, hist, 64 * 4)) != NQM_OK) return rc;
-        // stat: double sum of 2^pred, u32 count over the threshold, float threshold
-        if ((rc = ensure_dev(ctx, stat, 32)) != NQM_OK) return rc;
-        double* const d_psum = static_cast<double*>(stat.ptr);
+    std::vector<uint32_t> split_cap, split_out, goal_cap;  // sizes of the live run
+    std::vector<uint8_t> goal_rows(ns);                    // rows-goal target rows
+    const uint32_t learn_count = count;
+    const int learn_spare = spare;
+    auto learned_stage = [&](bool live, Item*& q, uint32_t*& slot, uint32_t& cnt, int& sp) -> int {
+        int r;
+        auto need = [&](DevBuf& d, size_t bytes) -> int {
+            if (live) return ensure_dev(ctx, d, bytes);
+            return d.cap >= bytes ? NQM_OK : fail(ctx, NQM_EDEVICE, "graph capture: buffer smaller than planned");
+        };
+        if ((r = need(b.hist, 64 * 4)) != NQM_OK) return r;
+        // stat: double sum, u32 over, float threshold; per-block partial sums from byte 64
+        if ((r = need(b.stat, 64 + (size_t)kPredSumBlocks * 8)) != NQM_OK) return r;
+        double* const d_psum = static_cast<double*>(b.stat.ptr);
         uint32_t* const d_over = reinterpret_cast<uint32_t*>(d_psum + 1);
         float* const d_thresh = reinterpret_cast<float*>(d_over + 1);
+        double* const d_bsums = d_psum + 8;
         uint32_t* const d_slot = d_cnt + 16;  // slots 16..31 for this stage
-        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
-        auto predict = [&](Item* items, uint32_t* cnt_slot, uint32_t n_items) -> int {
-            int r;
-            if ((r = ensure_dev(ctx, feats, (size_t)n_items * 32)) != NQM_OK) return r;
-            if ((r = ensure_dev(ctx, pred, (size_t)n_items * 4)) != NQM_OK) return r;
-            NQM_HIP(ctx, mlp_features_items(st, items, cnt_slot, feats.ptr, n_items));
-            return mlp_predict_device(t.model, feats.ptr, n_items, static_cast<float*>(pred.ptr));
+        auto predict = [&]() -> int {
+            int r2;
+            if ((r2 = need(b.feats, (size_t)cnt * 32)) != NQM_OK) return r2;
+            if ((r2 = need(b.pred, (size_t)cnt * 4)) != NQM_OK) return r2;
+            NQM_HIP(ctx, mlp_features_items(st, q, slot, b.feats.ptr, cnt));
+            return mlp_predict_device(t.model, b.feats.ptr, cnt, static_cast<float*>(b.pred.ptr));
         };
-        for (uint32_t r = 0; r <= rounds; ++r) {
-            if ((rc = predict(final_q, final_slot, count)) != NQM_OK) return rc;
-            if (r == rounds) break;
+        const uint32_t nsplit = live ? (t.split_rounds ? t.split_rounds : 1u) : (uint32_t)split_cap.size();
+        for (uint32_t i = 0;; ++i) {
+            if ((r = predict()) != NQM_OK) return r;
+            if (i == nsplit) break;
             // Split the heavy tail: items predicted above 8x the level's mean
             // (or the fixed threshold). The largest-first order handles the
             // rest, and the rows-goal expansion that follows divides every
             // item ~10^2-fold anyway; splitting nearly everything (a lane's
             // fair share as the bar) only added a full level of copies.
+            const float* const d_pred = static_cast<const float*>(b.pred.ptr);
             NQM_HIP(ctx, k2_fill_u32(st, reinterpret_cast<uint32_t*>(d_psum), 4, 0));
-            NQM_HIP(ctx, k2_split_threshold(st, static_cast<const float*>(pred.ptr), final_slot, count,
-                                            t.split_log2, 8.0 / std::max(count, 1u), d_psum, d_over, d_thresh));
-            uint32_t over = 0;
-            NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
-            NQM_HIP(ctx, hipStreamSynchronize(st));
-            if (over == 0) break;
-            const uint64_t cap64 = (uint64_t)count + (uint64_t)over * max_n;
-            if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
-            if ((rc = ensure_dev(ctx, b.q[spare], (size_t)cap64 * sizeof(Item))) != NQM_OK) return rc;
-            uint32_t* out_slot = d_slot + (r & 7);
-            const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
-            const float* const d_pred = static_cast<const float*>(pred.ptr);
-            NQM_HIP(ctx, k2_expand_count(st, grid, final_q, final_slot, d_part, out_slot, d_target, d_pred,
-                                         d_thresh));
-            NQM_HIP(ctx, k2_expand_write(st, grid, final_q, final_slot, d_part, false,
-                                         static_cast<Item*>(b.q[spare].ptr), (uint32_t)cap64, d_target, d_nodes,
-                                         d_exec, d_err, d_pred, d_thresh));
-            NQM_HIP(ctx, hipMemcpyAsync(&count, out_slot, 4, hipMemcpyDeviceToHost, st));
-            NQM_HIP(ctx, hipStreamSynchronize(st));
-            final_q = static_cast<Item*>(b.q[spare].ptr);
-            final_slot = out_slot;
-            spare ^= 1;
-            ++split_levels;
+            NQM_HIP(ctx, k2_split_threshold(st, d_pred, slot, cnt, t.split_log2, 8.0 / std::max(cnt, 1u), d_bsums,
+                                            d_psum, d_over, d_thresh));
+            uint32_t cap = 0;
+            if (live) {
+                uint32_t over = 0;
+                NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
+                NQM_HIP(ctx, hipStreamSynchronize(st));
+                if (over == 0) break;
+                const uint64_t cap64 = (uint64_t)cnt + (uint64_t)over * max_n;
+                if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
+                cap = (uint32_t)cap64;
+            } else {
+                cap = split_cap[i];
+            }
+            if ((r = need(b.q[sp], (size_t)cap * sizeof(Item))) != NQM_OK) return r;
+            uint32_t* const out_slot = d_slot + (i & 7);
+            const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (cnt + 255u) / 256u);
+            NQM_HIP(ctx, k2_expand_count(st, grid, q, slot, d_part, out_slot, d_target, d_pred, d_thresh));
+            NQM_HIP(ctx, k2_expand_write(st, grid, q, slot, d_part, false, static_cast<Item*>(b.q[sp].ptr), cap,
+                                         d_target, d_nodes, d_exec, d_err, d_pred, d_thresh));
+            if (live) {
+                NQM_HIP(ctx, hipMemcpyAsync(&cnt, out_slot, 4, hipMemcpyDeviceToHost, st));
+                NQM_HIP(ctx, hipStreamSynchronize(st));
+                split_cap.push_back(cap);
+                split_out.push_back(cnt);
+                ++split_levels;
+            } else {
+                cnt = split_out[i];
+            }
+            q = static_cast<Item*>(b.q[sp].ptr);
+            slot = out_slot;
+            sp ^= 1;
         }
         if (!(t.flags & NQM_TUNE_NO_ORDER)) {
-            if ((rc = ensure_dev(ctx, b.q[spare], (size_t)std::max(count, 1u) * sizeof(Item))) != NQM_OK)
-                return rc;
-            NQM_HIP(ctx, k2_order_by_prediction(st, final_q, static_cast<const float*>(pred.ptr), final_slot,
-                                                count, static_cast<uint32_t*>(hist.ptr),
-                                                static_cast<Item*>(b.q[spare].ptr)));
-            final_q = static_cast<Item*>(b.q[spare].ptr);
-            spare ^= 1;
+            if ((r = need(b.q[sp], (size_t)std::max(cnt, 1u) * sizeof(Item))) != NQM_OK) return r;
+            NQM_HIP(ctx, k2_order_by_prediction(st, q, static_cast<const float*>(b.pred.ptr), slot, cnt,
+                                                static_cast<uint32_t*>(b.hist.ptr),
+                                                static_cast<Item*>(b.q[sp].ptr)));
+            q = static_cast<Item*>(b.q[sp].ptr);
+            sp ^= 1;
         }
         // Then the stack-depth goal of the automatic depth: every item is
         // expanded to row n-1-kRowsGoal (split items may be deeper already).
         // Expansion keeps the queue's order, so the largest-predicted-first
-        // order carries over to the children.
-        if (auto_depth) {
-            std::vector<uint8_t> goal(ns);
+        // order carries over to the children. The target rows go to their
+        // own buffer: a graph's first levels still read the original ones.
+        if (!auto_depth) return NQM_OK;
+        if (live) {
             for (uint32_t s = 0; s < ns; ++s) {
                 const uint32_t n = solves[s].n;
                 const uint32_t g = n > kRowsGoal + 1u ? n - 1u - kRowsGoal : 0u;
-                goal[s] = (uint8_t)std::max<uint32_t>(final_row[s], std::min(g, n - 1u));
+                goal_rows[s] = (uint8_t)std::max<uint32_t>(final_row[s], std::min(g, n - 1u));
             }
-            NQM_HIP(ctx, hipMemcpyAsync(b.target.ptr, goal.data(), ns, hipMemcpyHostToDevice, st));
-            for (uint32_t L = 0;; ++L) {
+            if ((r = ensure_dev(ctx, b.goal, std::max<size_t>(ns, 1))) != NQM_OK) return r;
+            NQM_HIP(ctx, hipMemcpyAsync(b.goal.ptr, goal_rows.data(), ns, hipMemcpyHostToDevice, st));
+        }
+        const uint8_t* const d_goal = static_cast<const uint8_t*>(b.goal.ptr);
+        for (uint32_t L = 0;; ++L) {
+            if (live) {
                 bool below = false;
-                for (uint32_t s = 0; s < ns; ++s) below |= final_row[s] < goal[s];
-                if (!below || count == 0) break;
-                uint32_t* out_slot = d_cnt + 24 + (L & 3);
-                const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
-                NQM_HIP(ctx, k2_expand_count(st, grid, final_q, final_slot, d_part, out_slot, d_target));
-                uint32_t next = 0;
+                for (uint32_t s = 0; s < ns; ++s) below |= final_row[s] < goal_rows[s];
+                if (!below || cnt == 0) break;
+            } else if (L == goal_cap.size()) {
+                break;
+            }
+            uint32_t* const out_slot = d_cnt + 24 + (L & 3);
+            const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (cnt + 255u) / 256u);
+            NQM_HIP(ctx, k2_expand_count(st, grid, q, slot, d_part, out_slot, d_goal));
+            uint32_t next = 0;
+            if (live) {
                 NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
                 NQM_HIP(ctx, hipStreamSynchronize(st));
                 if (next > kAutoMaxItems) break;
-                if ((rc = ensure_dev(ctx, b.q[spare], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK)
-                    return rc;
-                NQM_HIP(ctx, k2_expand_write(st, grid, final_q, final_slot, d_part, false,
-                                             static_cast<Item*>(b.q[spare].ptr), next, d_target, d_nodes, d_exec,
-                                             d_err));
-                final_q = static_cast<Item*>(b.q[spare].ptr);
-                final_slot = out_slot;
-                spare ^= 1;
-                count = next;
-                for (uint32_t s = 0; s < ns; ++s)
-                    if (final_row[s] < goal[s]) final_row[s]++;
+                goal_cap.push_back(next);
+            } else {
+                next = goal_cap[L];
             }
+            if ((r = need(b.q[sp], (size_t)std::max(next, 1u) * sizeof(Item))) != NQM_OK) return r;
+            NQM_HIP(ctx, k2_expand_write(st, grid, q, slot, d_part, false, static_cast<Item*>(b.q[sp].ptr), next,
+                                         d_goal, d_nodes, d_exec, d_err));
+            q = static_cast<Item*>(b.q[sp].ptr);
+            slot = out_slot;
+            sp ^= 1;
+            cnt = next;
+            if (live)
+                for (uint32_t s = 0; s < ns; ++s)
+                    if (final_row[s] < goal_rows[s]) final_row[s]++;
         }
+        return NQM_OK;
+    };
+    if (t.model && count) {
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        if ((rc = learned_stage(true, final_q, final_slot, count, spare)) != NQM_OK) return rc;
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
         NQM_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
         NQM_HIP(ctx, hipEventElapsedTime(&learn_ms, ctx->ev_start, ctx->ev_stop));
@@ -519,13 +556,16 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     p.item_count = d_qcount;
 
     float run_ms = 0.f;
-    uint32_t launches = levels, rounds = 0;
+    // expansion levels (the learned stage's included) and drain rounds; a
+    // graph counts one launch
+    uint32_t launches = levels + split_levels + (uint32_t)goal_cap.size(), rounds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 
     if (use_graph) {
-        // One graph: clears, every expand level, the drain. Replays the whole
-        // batch from the initial items with the level sizes learned above.
+        // One graph: clears, every expand level, the learned stage, the
+        // drain. Replays the whole batch from the initial items with the
+        // sizes learned above.
         p.items = final_q;
         p.spill = nullptr;
         p.spill_cap = 0;
@@ -535,14 +575,28 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (ins[L] + 255u) / 256u);
             e = k2_expand_count(st, grid, level_in(L), d_cnt + L, d_part, d_cnt + L + 1, d_target);
             if (e == hipSuccess)
-                e = k2_expand_write(st, grid, level_in(L), d_cnt + L, d_part, true, level_out(L), caps[L],
+                e = k2_expand_write(st, grid, level_in(L), d_cnt + L, d_part, !t.model, level_out(L), caps[L],
                                     d_target, d_nodes, d_exec, d_err);
         }
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(d_qcount, d_cnt + levels, 4, hipMemcpyDeviceToDevice, st);
-        if (e == hipSuccess) e = k2_fill_u32(st, d_head, kHeadSegs * kHeadStride, 0);
-        if (e == hipSuccess) e = k2_drain(st, blocks, threads, p);
+        int lrc = NQM_OK;
+        uint32_t* queue_slot = d_cnt + levels;
+        if (e == hipSuccess && t.model && learn_count) {
+            Item* q = level_in(levels);
+            uint32_t cnt = learn_count;
+            int sp = learn_spare;
+            lrc = learned_stage(false, q, queue_slot, cnt, sp);
+            if (lrc == NQM_OK && q != final_q)
+                lrc = fail(ctx, NQM_EDEVICE, "graph capture: the learned stage left another queue than planned");
+        }
+        if (e == hipSuccess && lrc == NQM_OK)
+            e = hipMemcpyAsync(d_qcount, queue_slot, 4, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && lrc == NQM_OK) e = k2_fill_u32(st, d_head, kHeadSegs * kHeadStride, 0);
+        if (e == hipSuccess && lrc == NQM_OK) e = k2_drain(st, blocks, threads, p);
         hipError_t e2 = hipStreamEndCapture(st, &graph);
+        if (lrc != NQM_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return lrc;
+        }
         if (e != hipSuccess || e2 != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
             return fail(ctx, NQM_EDEVICE, "graph capture failed: %s / %s", hipGetErrorString(e),
@@ -556,7 +610,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
         NQM_HIP(ctx, hipGraphLaunch(gexec, st));
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-        launches = 1;
+        ++launches;  // the planning levels ran live, then the graph
         rounds = 1;
         // keep it: the next identical call launches it again
         b.plan.key = key;

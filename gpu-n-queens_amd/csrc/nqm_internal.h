@@ -46,7 +46,7 @@ struct BatchWork {
     // solutions and u64 nodes (one copy brings every result back)
     DevBuf init, q[2], counts, head, spill_count, target, res, part, wave_part;
     DevBuf cont[2], cont_counts;     // parked pieces of budget rounds
-    DevBuf feats, pred, hist, stat;  // learned splitting
+    DevBuf feats, pred, hist, stat, goal;  // learned splitting (goal: rows-goal target rows)
     GraphPlan plan;
 };
 void drop_plan(GraphPlan& plan);
@@ -174,9 +174,13 @@ hipError_t k2_expand_write(hipStream_t s, uint32_t grid, const Item* in, const u
                            const float* pred = nullptr, const float* thresh = nullptr);
 // Split threshold of the learned stage, on the device: thresh = fixed if
 // fixed > 0, else log2(max(1, sum(2^pred) * scale)); over = #(pred > thresh).
-// stats[0..1] (double sum) and stats[2] (u32 over) must be zero on entry.
+// The sum is formed in a fixed order (block_sums: kPredSumBlocks doubles of
+// scratch), so equal inputs give equal thresholds. over must be zero on
+// entry. Device counts are read up to max_count.
+constexpr uint32_t kPredSumBlocks = 1024;
 hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
-                              float fixed, double scale, double* sum, uint32_t* over, float* thresh);
+                              float fixed, double scale, double* block_sums, double* sum, uint32_t* over,
+                              float* thresh);
 hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
                                   uint32_t max_count, uint32_t* hist, Item* out);
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
@@ -188,6 +192,9 @@ int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
 
 // ---- subtree-size predictor (mlp.hip) -------------------------------------
 int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y);
+// Changes whenever the model's weights change (process-wide counter, so a
+// new model never repeats an old model's value).
+uint64_t mlp_version(const nqm_mlp* mlp);
 hipError_t mlp_features_items(hipStream_t s, const Item* items, const uint32_t* count, void* feats,
                               uint32_t max_count);
 

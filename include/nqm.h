@@ -47,7 +47,9 @@ typedef struct {
     uint64_t nodes;        /* placements executed below the prefixes */
     double kernel_ms;      /* device time of the counting launch(es) */
     double wall_ms;        /* host wall time of the whole call */
-    uint32_t launches;     /* kernel launches the call made */
+    uint32_t launches;     /* expansion levels and drain rounds the call ran (a
+                              graph launch counts one; a graph-mode call that
+                              builds its graph also ran the planning levels) */
     uint32_t blocks;       /* workgroups of the (first) counting launch */
     double search_ms;      /* device time of the search launches alone (K1 count;
                               K2 drain rounds), without expansion or splitting */
@@ -188,9 +190,10 @@ typedef struct {
 
 #define NQM_TUNE_NO_MIRROR 0x1u  /* do not halve empty boards by mirror symmetry */
 #define NQM_TUNE_NO_DONATE 0x2u  /* no in-wave work donation (queue refill only) */
-#define NQM_TUNE_GRAPH 0x4u      /* run expand + drain as one captured HIP graph; a
-                                    call repeating the previous call's solves and
-                                    tuning launches that graph again */
+#define NQM_TUNE_GRAPH 0x4u      /* run expand (+ learned stage) + drain as one
+                                    captured HIP graph; a call repeating the
+                                    previous call's solves, tuning and model
+                                    weights launches that graph again */
 #define NQM_TUNE_NO_ORDER 0x8u   /* with a model: split, but keep the queue order */
 
 struct nqm_mlp;

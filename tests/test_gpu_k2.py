@@ -103,6 +103,7 @@ def test_graph_plan_replays_and_is_dropped(ctx, known_totals):
         c, nd, _ = ctx.solve_batch(solves)
         eager[name] = (list(c), list(nd))
     assert eager["a"][0][0] == known_totals[15] and eager["b"][0][0] == known_totals[13]
+    prev = None
     for name in ("a", "a", "a", "b", "a", "b", "b", "plain-a", "a", "a"):
         solves = a if name.endswith("a") else b
         flags = 0 if name.startswith("plain") else nqm.TUNE_GRAPH
@@ -110,6 +111,9 @@ def test_graph_plan_replays_and_is_dropped(ctx, known_totals):
         key = "a" if name.endswith("a") else "b"
         assert (list(c), list(nd)) == eager[key], name
         assert st.total == sum(eager[key][0])
+        # a replay is one launch; building the graph also ran the planning levels
+        assert (st.launches == 1) == (name == prev), (name, prev, st.launches)
+        prev = name
 
 
 def test_full_and_dead_prefixes(ctx):

@@ -51,6 +51,51 @@ def test_model_generalises_to_a_larger_board(ctx, small_model):
     assert e["top1pct_overlap"] > 0.2, e
 
 
-def test_graph_with_model_is_refused(ctx, small_model):
-    rc, *_ = ctx.solve_batch([(12, ())], raw=True, model=small_model, flags=nqm.TUNE_GRAPH)
-    assert rc == nqm.EUNSUPPORTED
+@pytest.mark.parametrize("kw", [dict(), dict(split_rounds=2, split_log2=6.0), dict(flags=nqm.TUNE_NO_ORDER)])
+def test_graph_with_model_replays_exactly(ctx, small_model, known_totals, kw):
+    """With a model the graph holds the learned stage too. Repeated calls
+    replay it (one launch) with eager GPU work and synchronizations between
+    the replays, and every replay matches the eager counts."""
+    solves = [(15, ()), (13, (1, 3)), (12, ())]
+    kw = dict(kw)
+    flags = kw.pop("flags", 0)
+    c0, n0, _ = ctx.solve_batch(solves, model=small_model, flags=flags, **kw)
+    assert int(c0[0]) == known_totals[15] and int(c0[2]) == known_totals[12]
+    for i in range(4):
+        c, nd, st = ctx.solve_batch(solves, model=small_model, flags=flags | nqm.TUNE_GRAPH, **kw)
+        assert list(c) == list(c0) and list(nd) == list(n0), (kw, i)
+        assert (st.launches == 1) == (i > 0), (kw, i, st.launches)
+        learned.prediction_error(ctx, small_model, 12, 4)  # eager work between replays
+
+
+def test_graph_with_model_follows_the_weights(ctx, small_model, known_totals, node_totals):
+    """A graph is keyed by the model's weights: new weights (even the same
+    values set again) rebuild it, and a new model is never taken for an old
+    one."""
+    m = nqm.Mlp(ctx, seed=5)
+    try:
+        p = small_model.get_params()
+        m.set_params(p)
+        solves = [(16, ())]
+
+        def run():
+            c, nd, st = ctx.solve_batch(solves, model=m, flags=nqm.TUNE_GRAPH)
+            assert int(c[0]) == known_totals[16] and int(nd[0]) == node_totals[16]
+            return st.launches
+
+        assert run() > 1 and run() == 1
+        m.set_params(p)  # same values, new weights version
+        assert run() > 1 and run() == 1
+        q = p.copy()
+        q[-1] += 3.0  # every prediction 8x larger: the split and ordering change
+        m.set_params(q)
+        assert run() > 1 and run() == 1
+    finally:
+        m.close()
+    m2 = nqm.Mlp(ctx, seed=5)  # may reuse the freed model's address
+    try:
+        m2.set_params(p)
+        c, nd, st = ctx.solve_batch([(16, ())], model=m2, flags=nqm.TUNE_GRAPH)
+        assert int(c[0]) == known_totals[16] and st.launches > 1
+    finally:
+        m2.close()
