@@ -202,7 +202,10 @@ typedef struct {
     uint32_t depth;        /* rows placed before the queue (per solve, capped
                               at n-1 and raised to k); 0 = automatic */
     uint32_t waves_per_cu; /* target resident waves per CU; 0 = as many as fit */
-    uint32_t threads;      /* workgroup size 64/128/256; 0 = 64 */
+    uint32_t threads;      /* workgroup size 64/128/256; 0 = automatic: the
+                              largest that keeps all but at most one of the
+                              waves per CU of one-wave workgroups (64 when
+                              waves_per_cu is set) */
     uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 8 */
     uint64_t budget;       /* passes per wave before unfinished pieces are parked
                               and the drain relaunched (restart); 0 = one launch */
