@@ -240,6 +240,7 @@ def _progress(label, every=100):
 
 
 def workload_serving(ctx, nqm, args):
+    import numpy as np
     from nqm import serving, workloads
     known = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
              17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884}  # test.cu:18
@@ -261,6 +262,14 @@ def workload_serving(ctx, nqm, args):
     finally:
         srv.close()
     summ = serving.latency_summary(res, wall)
+    # latency by board size: the tail belongs to the few large boards
+    by_n = {}
+    for n in sorted({n for _, n, _ in reqs}):
+        lat = [res[t][3] for t, (_, m, _) in enumerate(reqs) if m == n and t in res]
+        if lat:
+            by_n[n] = {"requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                       "p99_ms": float(np.percentile(lat, 99))}
+    summ["by_n"] = by_n
     bad = [t for t, (_, n, p) in enumerate(reqs) if not p and res[t][0] != known[n]]
     # prefix requests: the same request alone through the batch path
     alone = [t for t, (_, n, p) in enumerate(reqs) if p and n <= 16][:100]
