@@ -244,15 +244,37 @@ def workload_serving(ctx, nqm, args):
     from nqm import serving, workloads
     known = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
              17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884}  # test.cu:18
-    # capacity from a closed-loop run of the first 10% of the stream
-    probe = workloads.serving_stream(count=args.requests, rate=1.0)[: max(20, args.requests // 10)]
+    # Capacity. The stream's work sits in its few large boards (about 5% are
+    # N=20), so a sample's request rate says little about the whole stream's.
+    # A closed loop over every 10th request gives work per second (executed
+    # placements: half the full-tree count for a mirror-halved empty board);
+    # the stream's expected work per request comes from the sample's mean per
+    # (n, prefixed) class, weighted by the whole stream's classes.
+    base = workloads.serving_stream(count=args.requests, rate=1.0)  # same requests, unit rate
+    probe = base[::10]
     srv = ctx.server()
     try:
-        res, wall = serving.replay(srv, [(0.0, n, p) for _, n, p in probe],
-                                   on_result=_progress("capacity probe", 50))
+        pres, pwall = serving.replay(srv, [(0.0, n, p) for _, n, p in probe],
+                                     on_result=_progress("capacity probe", 50))
     finally:
         srv.close()
-    capacity = len(probe) / (wall / 1e3)
+
+    def work(n, p, nodes):
+        return nodes / 2 if not p and n >= 2 else nodes
+
+    cls = {}
+    for t, (_, n, p) in enumerate(probe):
+        cls.setdefault((n, bool(p)), []).append(work(n, p, pres[t][1]))
+    mean = {k: sum(v) / len(v) for k, v in cls.items()}
+
+    def expect(n, p):
+        if (n, bool(p)) in mean:
+            return mean[(n, bool(p))]
+        same_n = [m for (m2, _), m in mean.items() if m2 == n]  # class absent: its board size
+        return sum(same_n) / len(same_n) if same_n else 0.0
+
+    work_rate = sum(work(n, p, pres[t][1]) for t, (_, n, p) in enumerate(probe)) / (pwall / 1e3)
+    capacity = work_rate / (sum(expect(n, p) for _, n, p in base) / len(base))
     rate = args.load * capacity
     reqs = workloads.serving_stream(count=args.requests, rate=rate)
     srv = ctx.server()
@@ -283,7 +305,10 @@ def workload_serving(ctx, nqm, args):
             "data": "synthetic",
             "config": {"workload": f"{args.requests} Poisson requests, N 10..20 ~ 1/N^2, 30% with a "
                                    f"1-3 queen prefix", "rate_per_s": rate,
-                       "capacity_probe_per_s": capacity, "target_load": args.load},
+                       "capacity_per_s": capacity, "target_load": args.load,
+                       "probe": {"requests": len(probe), "sample": "every 10th request, closed loop",
+                                 "requests_per_s": len(probe) / (pwall / 1e3),
+                                 "placements_per_s": work_rate}},
             "latency": summ, "rounds": rounds,
             "checked": {"empty_boards_vs_table": sum(1 for _, n, p in reqs if not p),
                         "prefix_requests_vs_alone": len(alone), "mismatches": bad}}

