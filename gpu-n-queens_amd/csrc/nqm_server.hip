@@ -37,6 +37,10 @@ constexpr uint32_t kServeStackRows = 11;
 // on a closed loop of 200 mixed requests: 13.7 requests/s against 10.8 at
 // 16,384 passes; every round pays host round trips and a completion scan.
 constexpr uint64_t kServeBudget = 65536;
+// Default queue memory (each of the two queue buffers): room for N=20 staged
+// to row 8 (148 M items, 4.4 GiB), which takes 1.41 s alone through the server
+// against 1.99 s at row 7 (1.18 s as a batch).
+constexpr uint64_t kServeQueueBytes = 8ull << 30;
 
 __global__ void srv_zero_slots_kernel(const uint32_t* __restrict__ slots, uint32_t n, uint64_t* __restrict__ a,
                                       uint64_t* __restrict__ b) {
@@ -77,6 +81,7 @@ struct nqm_server {
     int cur = 0;
     int ci = 0;          // cont[ci] holds the parked pieces
     uint32_t conts = 0;  // parked pieces waiting for the next round
+    bool fresh = false;  // requests were put in front of the queue since the last round
     uint64_t qstart = 0;  // the queue is q[cur][qstart, qstart + qcount)
     uint32_t qcount = 0;
     struct Slot {
@@ -251,7 +256,7 @@ int nqm_server_create(nqm_ctx* ctx, const nqm_server_cfg* cfg, nqm_server** out)
         return fail(ctx, NQM_EINVAL, "unknown server flags");
     }
     s->slots_n = s->cfg.max_active ? std::min(s->cfg.max_active, 1u << 20) : 256u;
-    const uint64_t qbytes = s->cfg.queue_bytes ? s->cfg.queue_bytes : (1ull << 30);
+    const uint64_t qbytes = s->cfg.queue_bytes ? s->cfg.queue_bytes : kServeQueueBytes;
     s->cap_items = std::max<uint64_t>(qbytes / sizeof(Item), 1024);
     // headroom for one round of spills from every lane (31 rows at most)
     const uint64_t spill = (uint64_t)ctx->prop.multiProcessorCount * 64u * 32u * 32u;
@@ -349,6 +354,7 @@ int nqm_server_submit(nqm_server* s, const nqm_solve* reqs, const uint64_t* tags
                                         s->stage[which].ptr, (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice,
                                         st));
         if (front) s->qstart -= items;
+        s->fresh |= front && items != 0;
         s->qcount += items;
         s->free_slots.pop_back();
         sl.active = true;
@@ -429,9 +435,13 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         p.refill_idle = 8;
         p.donate = (s->cfg.flags & NQM_TUNE_NO_DONATE) ? 0u : 1u;
         p.budget = spill_ok ? budget : 0;
-        // new requests first, unless the parked pieces outnumber the lanes
-        // (then they go first, so that they cannot pile up)
-        p.items_first = s->conts <= lanes ? 1u : 0u;
+        // Queue items first (new requests are at its front), unless the
+        // parked pieces outnumber the lanes: then they go first, so that they
+        // cannot pile up. A round right after new requests went in front
+        // still takes the queue first while parked pieces stay under 4x the
+        // lanes; else a small request could wait rounds behind parked work.
+        p.items_first = (s->conts <= lanes || (s->fresh && s->conts <= 4 * lanes)) ? 1u : 0u;
+        s->fresh = false;
         NQM_HIP(ctx, k2_fill_u32(st, d_cc, 3, 0));
         NQM_HIP(ctx, k2_fill_u32(st, d_cc, 1, s->conts));
         p.cont_in = static_cast<const Cont*>(s->cont[s->ci].ptr);

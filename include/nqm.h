@@ -267,7 +267,8 @@ typedef struct {
                               row 6, or deep enough for at most 11 stack rows
                               (n > 18), as far as the queue memory left allows */
     uint64_t budget;       /* passes per wave per round; 0 = 65536 */
-    uint64_t queue_bytes;  /* device memory for queued work items; 0 = 1 GiB */
+    uint64_t queue_bytes;  /* device memory for queued work items (two buffers of
+                              this size are allocated); 0 = 8 GiB */
     uint32_t flags;        /* NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE */
     uint32_t reserved;
 } nqm_server_cfg;
