@@ -55,7 +55,8 @@ N = 18
 EXPECT = 666090624             # test.cu:18
 FULL_NODES = 59365844490       # full-tree placements at N=18
 HALF_NODES = FULL_NODES // 2   # executed with mirror halving (N even)
-OPS_PER_NODE = 12              # textbook u32 placement step
+OPS_PER_NODE = 12              # textbook u32 placement step: b=f&-f (2), f^=b (1), cols|b (1),
+                               # ((ld|b)<<1)&M (3), (rd|b)>>1 (2), ~(c|l|r)&M (3)
 
 
 def parse():

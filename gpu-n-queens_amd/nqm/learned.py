@@ -58,7 +58,7 @@ def train(ctx, X, Y, epochs=12, batch=4096, lr=2e-3, seed=20261203, val_frac=0.2
 
 def prediction_error(ctx, model, n, depth):
     """Predicted vs K1-counted log2 subtree sizes of the half pool of n at
-    `depth`: the judge of the predictor (no effect on any count)."""
+    `depth`: how the predictor is scored (no effect on any count)."""
     buf, cnt = ctx.pool_generate_device(n, depth, MIRROR_WEIGHT)
     try:
         _, _, pn = ctx.count_pool_device(n, depth, buf, cnt, per_prefix=True)
