@@ -950,13 +950,18 @@ __global__ void k2_bucket_scan_kernel(uint32_t* __restrict__ hist) {
 // One global atomic per (block, bucket): the block counts its items per
 // bucket in LDS, reserves each bucket's range once, then ranks its items
 // inside the range with LDS atomics. Order inside a bucket is arbitrary.
+// The ranks are dealt round-robin to nseg equal runs (rank r goes to run
+// r % nseg), so that each queue segment of the drain holds the same mix of
+// sizes, largest first, instead of one segment holding all the largest.
 __global__ void __launch_bounds__(256)
 k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ pred,
-                         const uint32_t* __restrict__ count, uint32_t max_count, uint32_t* __restrict__ cursor,
-                         Item* __restrict__ out) {
+                         const uint32_t* __restrict__ count, uint32_t max_count, uint32_t nseg,
+                         uint32_t* __restrict__ cursor, Item* __restrict__ out) {
     __shared__ uint32_t h[kBuckets];
     __shared__ uint32_t off[kBuckets];
     const uint32_t c = min(*count, max_count);
+    // run s holds the ranks r = s (mod nseg): q + (s < rm) of them, from s q + min(s, rm)
+    const uint32_t q = c / nseg, rm = c % nseg;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < c; b0 += gridDim.x * blockDim.x) {
         for (uint32_t i = threadIdx.x; i < kBuckets; i += blockDim.x) h[i] = 0;
         __syncthreads();
@@ -970,19 +975,23 @@ k2_bucket_scatter_kernel(const Item* __restrict__ in, const float* __restrict__ 
         for (uint32_t j = threadIdx.x; j < kBuckets; j += blockDim.x)
             off[j] = h[j] ? atomicAdd(&cursor[j], h[j]) : 0u;
         __syncthreads();
-        if (i < c) out[off[bk] + r] = in[i];
+        if (i < c) {
+            const uint32_t rank = off[bk] + r, s = rank % nseg;
+            out[s * q + min(s, rm) + rank / nseg] = in[i];
+        }
         __syncthreads();
     }
 }
 
 hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
-                                  uint32_t max_count, uint32_t* hist, Item* out) {
+                                  uint32_t max_count, uint32_t nseg, uint32_t* hist, Item* out) {
+    if (nseg == 0) return hipErrorInvalidValue;
     const uint32_t blocks = std::max(1u, std::min(2048u, (max_count + 255u) / 256u));
     hipLaunchKernelGGL(k2_fill_u32_kernel, dim3(1), dim3(64), 0, s, hist, kBuckets, 0u);
     hipLaunchKernelGGL(k2_bucket_hist_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, hist);
     hipLaunchKernelGGL(k2_bucket_scan_kernel, dim3(1), dim3(64), 0, s, hist);
-    hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, max_count, hist,
-                       out);
+    hipLaunchKernelGGL(k2_bucket_scatter_kernel, dim3(blocks), dim3(256), 0, s, in, pred, count, max_count, nseg,
+                       hist, out);
     return hipGetLastError();
 }
 

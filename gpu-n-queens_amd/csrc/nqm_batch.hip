@@ -491,7 +491,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         if (!(t.flags & NQM_TUNE_NO_ORDER)) {
             if ((r = need(b.q[sp], (size_t)std::max(cnt, 1u) * sizeof(Item))) != NQM_OK) return r;
             NQM_HIP(ctx, k2_order_by_prediction(st, q, static_cast<const float*>(b.pred.ptr), slot, cnt,
-                                                static_cast<uint32_t*>(b.hist.ptr),
+                                                t.budget ? 1u : kHeadSegs, static_cast<uint32_t*>(b.hist.ptr),
                                                 static_cast<Item*>(b.q[sp].ptr)));
             q = static_cast<Item*>(b.q[sp].ptr);
             sp ^= 1;
@@ -525,9 +525,23 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             NQM_HIP(ctx, k2_expand_count(st, grid, q, slot, d_part, out_slot, d_goal));
             uint32_t next = 0;
             if (live) {
-                NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
-                NQM_HIP(ctx, hipStreamSynchronize(st));
-                if (next > kAutoMaxItems) break;
+                // As in the plain expansion, a single solve's level that fits
+                // kAutoMaxItems at its bound (n - row children per item) is
+                // sized by the bound, with no round trip; the drain then reads
+                // the queue size on the device. Restart rounds need it on the host.
+                uint32_t grow = 1;
+                for (uint32_t s = 0; s < ns; ++s)
+                    if (final_row[s] < goal_rows[s]) grow = std::max(grow, solves[s].n - final_row[s]);
+                const uint64_t bound = (uint64_t)cnt * grow;
+                if (ns == 1 && !t.budget && bound <= kAutoMaxItems) {
+                    next = (uint32_t)bound;
+                    exact = false;
+                } else {
+                    NQM_HIP(ctx, hipMemcpyAsync(&next, out_slot, 4, hipMemcpyDeviceToHost, st));
+                    NQM_HIP(ctx, hipStreamSynchronize(st));
+                    if (next > kAutoMaxItems) break;
+                    exact = true;
+                }
                 goal_cap.push_back(next);
             } else {
                 next = goal_cap[L];

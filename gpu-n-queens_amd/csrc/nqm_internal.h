@@ -181,8 +181,10 @@ constexpr uint32_t kPredSumBlocks = 1024;
 hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
                               float fixed, double scale, double* block_sums, double* sum, uint32_t* over,
                               float* thresh);
+// Largest predicted first, dealt round-robin to nseg runs of equal size
+// (nseg = the drain's queue segments).
 hipError_t k2_order_by_prediction(hipStream_t s, const Item* in, const float* pred, const uint32_t* count,
-                                  uint32_t max_count, uint32_t* hist, Item* out);
+                                  uint32_t max_count, uint32_t nseg, uint32_t* hist, Item* out);
 hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2DrainParams& p);
 // Queue items one drain wave takes per atomic on the queue head (at most);
 // a budget round can hand back up to this many unstarted items per wave.
