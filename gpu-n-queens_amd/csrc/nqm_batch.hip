@@ -368,8 +368,11 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             }
         if (!any || count == 0) break;
         const uint64_t bound = (uint64_t)count * grow;
-        const bool blind =
-            ns == 1 && bound <= kAutoMaxItems && (!auto_depth || (!t.model && rows_left > kRowsGoal));
+        // with a model, a level whose input (even as an upper bound) is under
+        // want_items is needed whatever its size
+        const bool blind = ns == 1 && bound <= kAutoMaxItems &&
+                           (!auto_depth || (!t.model && rows_left > kRowsGoal) ||
+                            (t.model && (uint64_t)count < want_items));
         if (!blind && !exact) {  // the decision below needs the level's true size
             NQM_HIP(ctx, hipMemcpyAsync(&count, d_cnt + L, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
@@ -459,11 +462,18 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
                                             d_psum, d_over, d_thresh));
             uint32_t cap = 0;
             if (live) {
-                uint32_t over = 0;
-                NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
-                NQM_HIP(ctx, hipStreamSynchronize(st));
-                if (over == 0) break;
-                const uint64_t cap64 = (uint64_t)cnt + (uint64_t)over * max_n;
+                // Sized for every item splitting (n children at most) while
+                // that stays under 1 GiB, so the host need not wait for the
+                // threshold (with nothing over it the level is a copy); else
+                // sized by the items over the threshold.
+                uint64_t cap64 = (uint64_t)cnt * max_n;
+                if (cap64 * sizeof(Item) > (1ull << 30)) {
+                    uint32_t over = 0;
+                    NQM_HIP(ctx, hipMemcpyAsync(&over, d_over, 4, hipMemcpyDeviceToHost, st));
+                    NQM_HIP(ctx, hipStreamSynchronize(st));
+                    if (over == 0) break;
+                    cap64 = (uint64_t)cnt + (uint64_t)over * max_n;
+                }
                 if (cap64 > 0x7fffffffull) return fail(ctx, NQM_ERANGE, "split exceeds 2^31 items");
                 cap = (uint32_t)cap64;
             } else {
