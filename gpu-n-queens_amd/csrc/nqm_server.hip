@@ -84,6 +84,15 @@ struct nqm_server {
     bool fresh = false;  // requests were put in front of the queue since the last round
     uint64_t qstart = 0;  // the queue is q[cur][qstart, qstart + qcount)
     uint32_t qcount = 0;
+    // Which requests still have queued items, without reading the queue: the
+    // queue holds only requests' staged blocks (contiguous) and the items
+    // rounds handed back (appended at the tail). Positions here are logical,
+    // physical index in q[cur] + shift (moving the live range keeps them).
+    // A block is used up once the head has passed its end; handed-back
+    // ranges are kept until the head passes them, and only they and the
+    // parked pieces are scanned.
+    uint64_t shift = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> handed_back;
     struct Slot {
         bool active = false;
         uint64_t tag = 0;
@@ -91,6 +100,8 @@ struct nqm_server {
         uint64_t host_sols = 0, host_nodes = 0;
         uint32_t rounds = 0;
         double admit_ms = 0;
+        uint64_t blk_end = 0;   // logical end of the staged block
+        bool blk_done = true;   // the head has passed blk_end
     };
     std::vector<Slot> slots;
     std::vector<uint32_t> free_slots;
@@ -223,6 +234,7 @@ int make_room(nqm_server* s, uint64_t extra) {
     if (s->qcount)
         NQM_HIP(ctx, hipMemcpyAsync(s->q[s->cur ^ 1].ptr, static_cast<Item*>(s->q[s->cur].ptr) + s->qstart,
                                     (size_t)s->qcount * sizeof(Item), hipMemcpyDeviceToDevice, ctx->stream));
+    s->shift += s->qstart;  // logical positions stay
     s->cur ^= 1;
     s->qstart = 0;
     return NQM_OK;
@@ -349,10 +361,12 @@ int nqm_server_submit(nqm_server* s, const nqm_solve* reqs, const uint64_t* tags
         const bool front = items <= s->qstart;
         if (!front && (rc = make_room(s, items)) != NQM_OK) return rc;
         Item* const q0 = static_cast<Item*>(s->q[s->cur].ptr);
+        const uint64_t blk_lo = front ? s->qstart - items : s->qstart + s->qcount;  // physical
         if (items)
-            NQM_HIP(ctx, hipMemcpyAsync(front ? q0 + s->qstart - items : q0 + s->qstart + s->qcount,
-                                        s->stage[which].ptr, (size_t)items * sizeof(Item), hipMemcpyDeviceToDevice,
-                                        st));
+            NQM_HIP(ctx, hipMemcpyAsync(q0 + blk_lo, s->stage[which].ptr, (size_t)items * sizeof(Item),
+                                        hipMemcpyDeviceToDevice, st));
+        sl.blk_end = blk_lo + items + s->shift;
+        sl.blk_done = items == 0;
         if (front) s->qstart -= items;
         s->fresh |= front && items != 0;
         s->qcount += items;
@@ -459,8 +473,23 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         NQM_HIP(ctx, hipStreamSynchronize(st));
         if (err) return fail(ctx, NQM_ERANGE, "server queue capacity exceeded (err=0x%x)", err);
         const uint32_t h0 = std::min(hd, s->qcount);  // items [h0, qcount) were never taken
+        if (end > s->qcount) {  // handed back after the old tail
+            const uint64_t lo = s->qstart + s->qcount + s->shift, hi = s->qstart + end + s->shift;
+            if (!s->handed_back.empty() && s->handed_back.back().second == lo)
+                s->handed_back.back().second = hi;
+            else
+                s->handed_back.emplace_back(lo, hi);
+        }
         s->qstart += h0;
         s->qcount = end - h0;
+        // what the head has passed is used up for good
+        const uint64_t head = s->qstart + s->shift;
+        size_t keep = 0;
+        for (auto& r : s->handed_back)
+            if (r.second > head) s->handed_back[keep++] = {std::max(r.first, head), r.second};
+        s->handed_back.resize(keep);
+        for (auto& sl : s->slots)
+            if (sl.active && !sl.blk_done && sl.blk_end <= head) sl.blk_done = true;
         // parked pieces no wave resumed this round follow the new ones
         const uint32_t c0 = std::min(cc[1], s->conts);
         if (c0 < s->conts)
@@ -478,10 +507,11 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
                                static_cast<uint32_t*>(s->outst.ptr));
             NQM_HIP(ctx, hipGetLastError());
         }
-        if (s->qcount) {
-            const uint32_t grid = std::min<uint32_t>(4096u, (s->qcount + 255u) / 256u);
+        for (const auto& r : s->handed_back) {  // the only queued items not in a staged block
+            const uint32_t c = (uint32_t)(r.second - r.first);
+            const uint32_t grid = std::min<uint32_t>(4096u, (c + 255u) / 256u);
             hipLaunchKernelGGL(srv_queued_kernel, dim3(grid), dim3(256), 0, st,
-                               static_cast<const Item*>(s->q[s->cur].ptr) + s->qstart, s->qcount,
+                               static_cast<const Item*>(s->q[s->cur].ptr) + (r.first - s->shift), c,
                                static_cast<uint32_t*>(s->outst.ptr));
             NQM_HIP(ctx, hipGetLastError());
         }
@@ -500,7 +530,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
             nqm_server::Slot& sl = s->slots[i];
             if (!sl.active) continue;
             ++sl.rounds;
-            if ((s->qcount != 0 || s->conts != 0) && outst[i] != 0) continue;
+            if ((s->qcount != 0 || s->conts != 0) && (outst[i] != 0 || !sl.blk_done)) continue;
             nqm_result r{};
             r.tag = sl.tag;
             r.count = hs[i] + sl.host_sols;
