@@ -159,3 +159,18 @@ def test_trained_model_gains_over_splitting_without_it(ctx, small_model):
     assert t[0] <= w[0] and t[0] <= u[0], g
     assert t[1] < w[1] and t[1] < u[1], g
     assert w[1] - t[1] >= 0.5 * (w[1] - c[1]), g
+
+
+def test_learned_n18_n19_exact(ctx, small_model, known_totals, node_totals):
+    """The learned-splitting configuration at its own boards: N=18 and N=19
+    empty boards with the model, eagerly and as a replayed graph. N=18's node
+    total is pinned by the independent DFS; N=19's must equal the count
+    without a model (splitting moves work, never changes it)."""
+    c, plain_nodes, _ = ctx.solve_batch([(19, ())])
+    assert int(c[0]) == known_totals[19]
+    for n in (18, 19):
+        for flags in (0, nqm.TUNE_GRAPH, nqm.TUNE_GRAPH):  # the second graph call replays
+            c, nd, _ = ctx.solve_batch([(n, ())], model=small_model, flags=flags)
+            assert int(c[0]) == known_totals[n], (n, flags)
+            want = node_totals[n] if n == 18 else int(plain_nodes[0])
+            assert int(nd[0]) == want, (n, flags)
