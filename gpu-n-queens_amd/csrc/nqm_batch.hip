@@ -373,7 +373,9 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         const bool blind = ns == 1 && bound <= kAutoMaxItems &&
                            (!auto_depth || (!t.model && rows_left > kRowsGoal) ||
                             (t.model && (uint64_t)count < want_items));
-        if (!blind && !exact) {  // the decision below needs the level's true size
+        // the decision below needs the level's true size only where the size
+        // decides it (with a model, or within the rows goal)
+        if (!blind && !exact && auto_depth && (t.model || rows_left <= kRowsGoal)) {
             NQM_HIP(ctx, hipMemcpyAsync(&count, d_cnt + L, 4, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
             exact = true;
