@@ -35,7 +35,7 @@ namespace nqm {
 constexpr uint32_t kServeStackRows = 11;
 // Default passes per wave per round (about 9 ms at full occupancy). Measured
 // on a closed loop of 200 mixed requests: 13.7 requests/s against 10.8 at
-// 16,384 passes; every round pays host round trips and a completion scan.
+// 16,384 passes; every round pays host round trips and the completion flags.
 constexpr uint64_t kServeBudget = 65536;
 // Default queue memory (each of the two queue buffers): room for N=20 staged
 // to row 8 (148 M items, 4.4 GiB), which takes 1.41 s alone through the server
@@ -407,7 +407,9 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         uint32_t blocks = 0, threads = 0;
         {
             // same grid rule as the batch path
-            uint32_t th = 64;  // one-wave workgroups pack LDS stacks most tightly
+            // one-wave workgroups: the spill bound below counts one wave per
+            // workgroup
+            uint32_t th = 64;
             int per_cu = 0;
             if ((rc = k2_max_blocks_per_cu(th, k2_lds_bytes(rows, th), &per_cu)) != NQM_OK)
                 return fail(ctx, rc, "occupancy query failed");
