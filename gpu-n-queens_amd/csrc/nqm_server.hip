@@ -403,12 +403,10 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
     int rc;
     if (s->qcount || s->conts) {
         const uint32_t rows = stack_rows_needed(s);
-        nqm_tune t{};
         uint32_t blocks = 0, threads = 0;
         {
-            // same grid rule as the batch path
-            // one-wave workgroups: the spill bound below counts one wave per
-            // workgroup
+            // as many one-wave workgroups per CU as LDS allows (the spill
+            // bound below counts one wave per workgroup)
             uint32_t th = 64;
             int per_cu = 0;
             if ((rc = k2_max_blocks_per_cu(th, k2_lds_bytes(rows, th), &per_cu)) != NQM_OK)
@@ -416,7 +414,6 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
             blocks = (uint32_t)std::max(per_cu, 1) * (uint32_t)ctx->prop.multiProcessorCount;
             threads = th;
         }
-        (void)t;
         // a round hands back the unstarted items its waves hold (unfinished
         // pieces are parked as continuations); the queue itself stays in place
         const uint64_t spill_bound = (uint64_t)blocks * kGrabItems;
