@@ -1032,16 +1032,16 @@ k2_pred_total_kernel(const double* __restrict__ block_sums, uint32_t blocks, dou
     if (threadIdx.x == 0) *sum = t;
 }
 
-__device__ __forceinline__ float split_thresh(float fixed, double scale, const double* sum) {
-    return fixed > 0.f ? fixed : (float)log2(fmax(1.0, *sum * scale));
+__device__ __forceinline__ float split_thresh(float fixed, double mult, const double* sum, uint32_t count) {
+    return fixed > 0.f ? fixed : (float)log2(fmax(1.0, *sum * mult / (double)max(count, 1u)));
 }
 
 __global__ void __launch_bounds__(256)
 k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__ count, uint32_t max_count,
-                    float fixed, double scale, const double* __restrict__ sum, uint32_t* __restrict__ over,
+                    float fixed, double mult, const double* __restrict__ sum, uint32_t* __restrict__ over,
                     float* __restrict__ thresh) {
     const uint32_t c = min(*count, max_count);
-    const float t = split_thresh(fixed, scale, sum);
+    const float t = split_thresh(fixed, mult, sum, c);
     if (blockIdx.x == 0 && threadIdx.x == 0) *thresh = t;
     uint32_t n = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < c; i += gridDim.x * blockDim.x)
@@ -1051,14 +1051,14 @@ k2_pred_over_kernel(const float* __restrict__ pred, const uint32_t* __restrict__
 }
 
 hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
-                              float fixed, double scale, double* block_sums, double* sum, uint32_t* over,
+                              float fixed, double mult, double* block_sums, double* sum, uint32_t* over,
                               float* thresh) {
     const uint32_t blocks = std::max(1u, std::min(kPredSumBlocks, (max_count + 255u) / 256u));
     if (!(fixed > 0.f)) {
         hipLaunchKernelGGL(k2_pred_sum_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, block_sums);
         hipLaunchKernelGGL(k2_pred_total_kernel, dim3(1), dim3(256), 0, s, block_sums, blocks, sum);
     }
-    hipLaunchKernelGGL(k2_pred_over_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, fixed, scale,
+    hipLaunchKernelGGL(k2_pred_over_kernel, dim3(blocks), dim3(256), 0, s, pred, count, max_count, fixed, mult,
                        sum, over, thresh);
     return hipGetLastError();
 }

@@ -100,7 +100,8 @@ struct MlpArgs {
     const _Float16* x;       // [count][kIn]
     const float* t;          // labels (backward only)
     const uint32_t* index;   // optional sample permutation
-    uint32_t count;          // samples in this call
+    uint32_t count;          // samples in this call (an upper bound when count_dev is set)
+    const uint32_t* count_dev;  // optional: the true sample count, on the device
     float inv_batch;         // 1 / samples contributing to the mean loss
     float* y;                // optional predictions [count]
     float* grad;             // fp32 gradient accumulators (kParams), backward only
@@ -147,11 +148,12 @@ __global__ void __launch_bounds__(64 * kMlpWaves) mlp_pass_kernel(MlpArgs a) {
     }
     for (int i = 0; i < 16; ++i) gb1[i] = gb2[i] = gw3[i] = 0.f;
 
-    const uint32_t ntiles = (a.count + 15u) / 16u;
+    const uint32_t count = a.count_dev ? min(a.count, *a.count_dev) : a.count;
+    const uint32_t ntiles = (count + 15u) / 16u;
     const uint32_t gw = blockIdx.x * kMlpWaves + wave, nw = gridDim.x * kMlpWaves;
     for (uint32_t tile = gw; tile < ntiles; tile += nw) {
         const uint32_t s = tile * 16u + (uint32_t)r16;  // this lane's sample (column)
-        const bool valid = s < a.count;
+        const bool valid = s < count;
         const uint32_t src = valid ? (a.index ? a.index[s] : s) : 0u;
         half4_t bx;
         for (int j = 0; j < 4; ++j) bx[j] = valid ? a.x[(size_t)src * kIn + q4 + j] : (_Float16)0.0f;
@@ -338,7 +340,8 @@ static uint64_t mix64(uint64_t& s) {  // splitmix64
 }
 
 static int mlp_launch_pass(nqm_mlp* mlp, const _Float16* x, const float* t, const uint32_t* index,
-                           uint32_t count, float inv_batch, float* y, bool backward) {
+                           uint32_t count, float inv_batch, float* y, bool backward,
+                           const uint32_t* count_dev = nullptr) {
     nqm_ctx* ctx = mlp->ctx;
     MlpArgs a;
     a.wh = static_cast<const _Float16*>(mlp->wh.ptr);
@@ -347,6 +350,7 @@ static int mlp_launch_pass(nqm_mlp* mlp, const _Float16* x, const float* t, cons
     a.t = t;
     a.index = index;
     a.count = count;
+    a.count_dev = count_dev;
     a.inv_batch = inv_batch;
     a.y = y;
     a.grad = static_cast<float*>(mlp->g.ptr);
@@ -368,9 +372,10 @@ static int mlp_launch_pass(nqm_mlp* mlp, const _Float16* x, const float* t, cons
     return NQM_OK;
 }
 
-int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y) {
+int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y, const uint32_t* d_count) {
     if (count == 0) return NQM_OK;
-    return mlp_launch_pass(mlp, static_cast<const _Float16*>(d_x), nullptr, nullptr, count, 0.f, d_y, false);
+    return mlp_launch_pass(mlp, static_cast<const _Float16*>(d_x), nullptr, nullptr, count, 0.f, d_y, false,
+                           d_count);
 }
 
 hipError_t mlp_features_items(hipStream_t s, const Item* items, const uint32_t* count, void* feats,

@@ -447,7 +447,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             if ((r2 = need(b.feats, (size_t)cnt * 32)) != NQM_OK) return r2;
             if ((r2 = need(b.pred, (size_t)cnt * 4)) != NQM_OK) return r2;
             NQM_HIP(ctx, mlp_features_items(st, q, slot, b.feats.ptr, cnt));
-            return mlp_predict_device(t.model, b.feats.ptr, cnt, static_cast<float*>(b.pred.ptr));
+            return mlp_predict_device(t.model, b.feats.ptr, cnt, static_cast<float*>(b.pred.ptr), slot);
         };
         const uint32_t nsplit = live ? (t.split_rounds ? t.split_rounds : 1u) : (uint32_t)split_cap.size();
         for (uint32_t i = 0;; ++i) {
@@ -460,8 +460,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             // fair share as the bar) only added a full level of copies.
             const float* const d_pred = static_cast<const float*>(b.pred.ptr);
             NQM_HIP(ctx, k2_fill_u32(st, reinterpret_cast<uint32_t*>(d_psum), 4, 0));
-            NQM_HIP(ctx, k2_split_threshold(st, d_pred, slot, cnt, t.split_log2, 8.0 / std::max(cnt, 1u), d_bsums,
-                                            d_psum, d_over, d_thresh));
+            NQM_HIP(ctx, k2_split_threshold(st, d_pred, slot, cnt, t.split_log2, 8.0, d_bsums, d_psum, d_over,
+                                            d_thresh));
             uint32_t cap = 0;
             if (live) {
                 // Sized for every item splitting (n children at most) while
@@ -488,8 +488,16 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             NQM_HIP(ctx, k2_expand_write(st, grid, q, slot, d_part, false, static_cast<Item*>(b.q[sp].ptr), cap,
                                          d_target, d_nodes, d_exec, d_err, d_pred, d_thresh));
             if (live) {
-                NQM_HIP(ctx, hipMemcpyAsync(&cnt, out_slot, 4, hipMemcpyDeviceToHost, st));
-                NQM_HIP(ctx, hipStreamSynchronize(st));
+                // The split level's size stays on the device (its kernels and
+                // the predictor read it there, cap sizes their launches),
+                // unless restart rounds need it on the host.
+                if (t.budget) {
+                    NQM_HIP(ctx, hipMemcpyAsync(&cnt, out_slot, 4, hipMemcpyDeviceToHost, st));
+                    NQM_HIP(ctx, hipStreamSynchronize(st));
+                } else {
+                    cnt = cap;
+                    exact = false;
+                }
                 split_cap.push_back(cap);
                 split_out.push_back(cnt);
                 ++split_levels;

@@ -173,13 +173,14 @@ hipError_t k2_expand_write(hipStream_t s, uint32_t grid, const Item* in, const u
                            const uint8_t* target_row, uint64_t* solve_nodes, uint64_t* exec_nodes, uint32_t* err,
                            const float* pred = nullptr, const float* thresh = nullptr);
 // Split threshold of the learned stage, on the device: thresh = fixed if
-// fixed > 0, else log2(max(1, sum(2^pred) * scale)); over = #(pred > thresh).
+// fixed > 0, else log2(max(1, mult * sum(2^pred) / count)), count read on the
+// device; over = #(pred > thresh).
 // The sum is formed in a fixed order (block_sums: kPredSumBlocks doubles of
 // scratch), so equal inputs give equal thresholds. over must be zero on
 // entry. Device counts are read up to max_count.
 constexpr uint32_t kPredSumBlocks = 1024;
 hipError_t k2_split_threshold(hipStream_t s, const float* pred, const uint32_t* count, uint32_t max_count,
-                              float fixed, double scale, double* block_sums, double* sum, uint32_t* over,
+                              float fixed, double mult, double* block_sums, double* sum, uint32_t* over,
                               float* thresh);
 // Largest predicted first, dealt round-robin to nseg runs of equal size
 // (nseg = the drain's queue segments).
@@ -193,7 +194,10 @@ size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
 
 // ---- subtree-size predictor (mlp.hip) -------------------------------------
-int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y);
+// count samples, or, with d_count, the device count of them (count is then
+// an upper bound that sizes the launch)
+int mlp_predict_device(nqm_mlp* mlp, const void* d_x, uint32_t count, float* d_y,
+                       const uint32_t* d_count = nullptr);
 // Changes whenever the model's weights change (process-wide counter, so a
 // new model never repeats an old model's value).
 uint64_t mlp_version(const nqm_mlp* mlp);
