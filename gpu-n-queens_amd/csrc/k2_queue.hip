@@ -1108,6 +1108,24 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     return hipGetLastError();
 }
 
+int k2_auto_threads(uint32_t stack_rows, uint32_t lds_per_block, uint32_t* threads) {
+    // resident workgroups per CU for workgroup size c (0: the stack does not fit)
+    auto blocks_per_cu = [&](uint32_t c, int* nb) -> int {
+        *nb = 0;
+        if (k2_lds_bytes(stack_rows, c) > lds_per_block) return NQM_OK;
+        return k2_max_blocks_per_cu(c, k2_lds_bytes(stack_rows, c), nb);
+    };
+    int rc, nb64 = 0;
+    if ((rc = blocks_per_cu(64, &nb64)) != NQM_OK) return rc;
+    *threads = 64;
+    for (uint32_t c : {128u, 256u}) {
+        int nb = 0;
+        if ((rc = blocks_per_cu(c, &nb)) != NQM_OK) return rc;
+        if (nb > 0 && nb * (int)(c / 64u) >= nb64 - 1) *threads = c;
+    }
+    return NQM_OK;
+}
+
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out) {
     int nb = 0;
     hipError_t e = threads == 64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k2_drain_kernel<64>, 64, lds_bytes)

@@ -151,28 +151,11 @@ void host_expand(const std::vector<uint8_t>& target, size_t max_items, HostInit*
 // requested cap allow.
 int drain_shape(nqm_ctx* ctx, uint32_t stack_rows, const nqm_tune& t, uint32_t* blocks,
                 uint32_t* threads) {
-    // resident workgroups per CU for workgroup size c (0: the stack does not fit)
-    auto blocks_per_cu = [&](uint32_t c, int* nb) -> int {
-        *nb = 0;
-        if (k2_lds_bytes(stack_rows, c) > ctx->lds_per_block) return NQM_OK;
-        return k2_max_blocks_per_cu(c, k2_lds_bytes(stack_rows, c), nb);
-    };
     int rc;
     uint32_t th = t.threads;
     if (th == 0 && t.waves_per_cu == 0) {
-        // Automatic: the largest workgroup that keeps all but at most one of
-        // the waves per CU that one-wave workgroups get. Measured: at 16 waves
-        // per CU, 256-lane workgroups beat one-wave workgroups at 17 by 1-3%
-        // (N=17, N=18, the 512-solve batch); at N=19's ten stack rows they
-        // would keep 12 of 15 and were 8% slower.
-        int nb64 = 0;
-        if ((rc = blocks_per_cu(64, &nb64)) != NQM_OK) return fail(ctx, rc, "occupancy query failed");
-        th = 64;
-        for (uint32_t c : {128u, 256u}) {
-            int nb = 0;
-            if ((rc = blocks_per_cu(c, &nb)) != NQM_OK) return fail(ctx, rc, "occupancy query failed");
-            if (nb > 0 && nb * (int)(c / 64u) >= nb64 - 1) th = c;
-        }
+        if ((rc = k2_auto_threads(stack_rows, ctx->lds_per_block, &th)) != NQM_OK)
+            return fail(ctx, rc, "occupancy query failed");
     } else if (th == 0) {
         th = 64;  // a wave budget per CU: one-wave workgroups meet it exactly
     }

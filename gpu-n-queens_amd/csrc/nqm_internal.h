@@ -192,6 +192,12 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
 constexpr uint32_t kGrabItems = 128;
 size_t k2_lds_bytes(uint32_t stack_rows, uint32_t threads);
 int k2_max_blocks_per_cu(uint32_t threads, size_t lds_bytes, int* out);
+// Drain workgroup size for a stack of stack_rows rows: the largest of
+// 64/128/256 lanes that keeps all but at most one of the waves per CU that
+// one-wave workgroups get. Measured: at 16 waves per CU, 256-lane workgroups
+// beat one-wave workgroups at 17 by 1-3% (N=17, N=18, a 512-solve batch); at
+// N=19's ten stack rows they would keep 12 of 15 and were 8% slower.
+int k2_auto_threads(uint32_t stack_rows, uint32_t lds_per_block, uint32_t* threads);
 
 // ---- subtree-size predictor (mlp.hip) -------------------------------------
 // count samples, or, with d_count, the device count of them (count is then

@@ -405,9 +405,10 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         const uint32_t rows = stack_rows_needed(s);
         uint32_t blocks = 0, threads = 0;
         {
-            // as many one-wave workgroups per CU as LDS allows (the spill
-            // bound below counts one wave per workgroup)
+            // as many workgroups per CU as LDS allows, of the automatic size
             uint32_t th = 64;
+            if ((rc = k2_auto_threads(rows, ctx->lds_per_block, &th)) != NQM_OK)
+                return fail(ctx, rc, "occupancy query failed");
             int per_cu = 0;
             if ((rc = k2_max_blocks_per_cu(th, k2_lds_bytes(rows, th), &per_cu)) != NQM_OK)
                 return fail(ctx, rc, "occupancy query failed");
@@ -416,7 +417,7 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         }
         // a round hands back the unstarted items its waves hold (unfinished
         // pieces are parked as continuations); the queue itself stays in place
-        const uint64_t spill_bound = (uint64_t)blocks * kGrabItems;
+        const uint64_t spill_bound = (uint64_t)blocks * (threads / 64u) * kGrabItems;
         const uint64_t lanes = (uint64_t)blocks * threads;
         if ((rc = ensure_dev(ctx, s->cont[s->ci], std::max<size_t>(s->conts, 1) * sizeof(Cont))) != NQM_OK)
             return rc;
