@@ -216,7 +216,9 @@ typedef struct {
     /* Learned splitting: with a model, the expansion stops early, every item
      * whose predicted log2(1 + nodes) exceeds split_log2 is split one row
      * deeper, up to split_rounds times (0 = 1), the queue is ordered largest
-     * predicted subtree first, and (automatic depth) the expansion then
+     * predicted subtree first (dealt round-robin to the drain's queue
+     * segments, so each starts with its share of the largest), and
+     * (automatic depth) the expansion then
      * continues to the stack-depth goal keeping that order. split_log2 = 0
      * picks log2(8 x the mean predicted size of the level). */
     struct nqm_mlp* model;
