@@ -50,6 +50,10 @@ def lib():
         L.orc_nqsf_write.argtypes = [ctypes.c_uint, ctypes.c_uint, U8P, ctypes.c_uint64,
                                      ctypes.c_uint64, U8P, ctypes.c_uint64]
         L.orc_nqsf_write.restype = ctypes.c_uint64
+        L.orc_sym_count.argtypes = [ctypes.c_uint, U64P, U64P, U64P]
+        L.orc_sym_count.restype = ctypes.c_int
+        L.orc_sym_brute.argtypes = [ctypes.c_uint, U64P, U64P]
+        L.orc_sym_brute.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -135,3 +139,22 @@ def nqsf_bytes(n, lock, pool, flags=3):
                              buf.ctypes.data_as(U8P), cap)
     assert w > 0
     return bytes(buf[:w])
+
+
+def sym_count(n):
+    """Symmetry-reduced count (nq_symmetry.c): (unique classes, total
+    solutions, placements of the reduced search)."""
+    u, t, nd = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().orc_sym_count(n, ctypes.byref(u), ctypes.byref(t), ctypes.byref(nd))
+    if rc != 0:
+        raise ValueError(f"orc_sym_count({n}) = {rc}")
+    return u.value, t.value, nd.value
+
+
+def sym_brute(n):
+    """Brute force over every solution's 8 images: (unique, total)."""
+    u, t = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().orc_sym_brute(n, ctypes.byref(u), ctypes.byref(t))
+    if rc != 0:
+        raise ValueError(f"orc_sym_brute({n}) = {rc}")
+    return u.value, t.value

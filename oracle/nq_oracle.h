@@ -74,6 +74,15 @@ int orc_lock_depth(unsigned n, uint64_t k, int mirror,
  * relative bitset encoding, independent of orc_count_prefix. */
 void orc_level_counts(unsigned n, unsigned max_k, uint64_t* levels);
 
+/* Symmetry-reduced counting (nq_symmetry.c; the reference's future work,
+ * README.md:38-45): solution classes under the 8 board symmetries (unique),
+ * the solutions they stand for (total) and the placements the reduced search
+ * makes (nodes). Pinned by orc_sym_brute and OEIS A002562. */
+int orc_sym_count(unsigned n, uint64_t* unique, uint64_t* total, uint64_t* nodes);
+/* Every solution and its 8 images; a class is counted at its smallest member
+ * (row by row). n <= 16. Returns -2 if the classes' orbits do not add up. */
+int orc_sym_brute(unsigned n, uint64_t* unique, uint64_t* total);
+
 /* Serialize a pool in the reference's .nqsf layout (nq_utils.cu:145-193)
  * into buf (capacity cap bytes). Returns bytes written or 0 if cap is too
  * small. flags is the 64-bit compile-flag word the header carries. */

@@ -308,3 +308,27 @@ def test_planted_mirror_mistake_is_caught(known_totals):
     wrong = sum(2 * s for c, s in enumerate(t) if c <= n // 2)
     right = sum((2 if c < n // 2 else 1) * s for c, s in enumerate(t))
     assert right == known_totals[n] and wrong != known_totals[n]
+
+
+# ---- symmetry-reduced counting (the reference's future work, README.md:38-45)
+
+def test_symmetry_reduction_matches_brute_force():
+    # every solution's 8 images, a class counted at its smallest member; the
+    # brute force also checks that the classes' orbit sizes add up to the total
+    for n in range(1, 11):
+        assert oracle.sym_count(n)[:2] == oracle.sym_brute(n), n
+
+
+def test_symmetry_reduction_published_counts(known_totals):
+    unique = {int(k): v for k, v in golden_json("unique_totals.json")["unique"].items()}
+    for n in range(1, 16):
+        u, t, nodes = oracle.sym_count(n)
+        assert u == unique[n], n
+        assert t == known_totals[n], n
+
+
+def test_symmetry_reduction_visits_less_than_the_mirror_halved_tree(node_totals):
+    # the point of the method: under half of the mirror-halved placements
+    for n in (12, 14):
+        _, _, nodes = oracle.sym_count(n)
+        assert nodes < 0.5 * node_totals[n] / 2, (n, nodes)
