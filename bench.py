@@ -469,6 +469,20 @@ def main():
     for v in ("persistent", "persistent_graph", "learned", "learned_graph"):
         if "plain" in variants and v in variants:
             line[f"{v}_vs_plain"] = variants["plain"]["ms_per_step"] / variants[v]["ms_per_step"]
+    if args.variant == "all":
+        # symmetry-reduced counting (the reference's future work): another
+        # method with its own placement count, so reported beside the
+        # variants, not among them
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            u, t, st = ctx.count_symmetric(N)
+            runs.append((1e3 * (time.perf_counter() - t0), st))
+            if t != EXPECT:
+                raise RuntimeError(f"symmetric total {t}")
+        w, st = min(runs, key=lambda x: x[0])
+        line["symmetry_reduced"] = {"ms": w, "kernel_ms": st.kernel_ms, "unique": u, "total": t,
+                                    "placements": st.nodes, "path": "K3 static kernel, first version"}
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)

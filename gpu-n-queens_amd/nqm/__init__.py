@@ -146,6 +146,7 @@ _SIGNATURES = {
     "nqm_state_size": ([ctypes.c_uint32], ctypes.c_size_t),
     "nqm_solve_batch": ([VP, ctypes.POINTER(Solve), ctypes.c_uint32, ctypes.POINTER(Tune), U64P,
                          U64P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_count_symmetric": ([VP, ctypes.c_uint32, U64P, U64P, ctypes.POINTER(Stats)], ctypes.c_int),
     "nqm_pool_count": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P],
                        ctypes.c_int),
     "nqm_pool_generate": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U8P,
@@ -362,6 +363,17 @@ def _solve_batch(ctx, solves, tune=None, raw=False, **tune_kw):
 
 
 Context.solve_batch = _solve_batch
+
+
+def _count_symmetric(ctx, n):
+    """Symmetry-reduced count of the empty n x n board: (unique classes,
+    total solutions, Stats)."""
+    u, t, st = ctypes.c_uint64(), ctypes.c_uint64(), Stats()
+    ctx._check(ctx.lib.nqm_count_symmetric(ctx.handle, n, ctypes.byref(u), ctypes.byref(t), ctypes.byref(st)))
+    return u.value, t.value, st
+
+
+Context.count_symmetric = _count_symmetric
 
 
 def _pool_count(ctx, n, lock, flags=0, per_col=False):

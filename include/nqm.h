@@ -255,6 +255,15 @@ int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
                  const nqm_tune_space* space, nqm_tune* best, nqm_tune_result* results,
                  uint32_t max_results, uint32_t* nresults);
 
+/* ---- symmetry-reduced counting --------------------------------------------
+ * No counterpart in the reference, which lists "transformation elimination"
+ * as future work (README.md:38-45). Counts the classes of solutions of the
+ * empty n x n board under its 8 symmetries (unique) and the solutions they
+ * stand for (total, equal to the reference's count), by the published corner
+ * / non-corner method: about 22% of the full tree's placements are made.
+ * stats->nodes = those placements (not comparable with the other paths). */
+int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats);
+
 /* ---- serving: continuous batching of count requests ---------------------
  * No reference counterpart. Requests join a device-resident work queue
  * between budgeted drain rounds (new requests are served first); pieces of
