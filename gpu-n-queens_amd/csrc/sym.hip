@@ -44,15 +44,20 @@ __host__ __device__ inline SymCase sym_case(uint32_t n, uint32_t b1, uint32_t co
 }
 
 // Candidates of row y under the case's rules (0: the row ends the branch).
+// The rules are selects, not branches (rows y < b1, y == b2 and y == last
+// are distinct: b1 < b2 < last), so lanes at different rows stay together.
 __host__ __device__ inline uint32_t sym_free(const SymCase& c, uint32_t y, uint32_t l, uint32_t d, uint32_t r) {
     uint32_t f = c.mask & ~(l | d | r);
-    if (c.corner) return (y < c.b1) ? (f & ~2u) : f;
-    if (y == c.last) return (f & c.lastmask) ? 0u : f;
-    if (y < c.b1) return f & ~c.side;
-    if (y == c.b2) {
+    // rows above b1: the corner case avoids column 1, the other the side columns
+    f &= (y < c.b1) ? ~(c.corner ? 2u : c.side) : ~0u;
+    if (!c.corner) {
+        // row b2: a side column left to the rows below ends the branch, one
+        // side column taken makes this row take the other
         const uint32_t ds = d & c.side;
-        if (!ds) return 0u;              // a side column would be left to the rows below
-        if (ds != c.side) f &= c.side;   // this row takes the other side column
+        const uint32_t at_b2 = ds == 0u ? 0u : (ds != c.side ? (f & c.side) : f);
+        f = y == c.b2 ? at_b2 : f;
+        // the last row avoids the columns within b1 of a corner
+        f = (y == c.last && (f & c.lastmask) != 0u) ? 0u : f;
     }
     return f;
 }
@@ -132,6 +137,7 @@ __host__ __device__ inline void sym_search(const SymItem& it, Stack& stack, Boar
     for (uint32_t row = 0; row < y0; ++row) board.set(row, it.col[row]);
     uint32_t y = y0, l = it.l, d = it.d, r = it.r;
     uint32_t f = sym_free(c, y, l, d, r);
+    uint32_t placed = 0;  // an item's subtree is at most (n - y0)! placements: rows left <= 11 fit
     for (;;) {
         if (f == 0u) {  // row exhausted: back to the row above (within the item)
             if (y == y0) break;
@@ -145,7 +151,7 @@ __host__ __device__ inline void sym_search(const SymItem& it, Stack& stack, Boar
         }
         const uint32_t bit = f & (0u - f);
         f ^= bit;
-        ++k.nodes;
+        ++placed;
         board.set(y, (uint32_t)__builtin_ctz(bit));
         if (y == c.last) {  // a full board (the last row has one free column at most)
             if (c.corner) {
@@ -165,6 +171,7 @@ __host__ __device__ inline void sym_search(const SymItem& it, Stack& stack, Boar
         ++y;
         f = sym_free(c, y, l, d, r);
     }
+    k.nodes += placed;
 }
 
 struct LdsStack {
