@@ -137,7 +137,7 @@ __host__ __device__ inline void sym_search(const SymItem& it, Stack& stack, Boar
     for (uint32_t row = 0; row < y0; ++row) board.set(row, it.col[row]);
     uint32_t y = y0, l = it.l, d = it.d, r = it.r;
     uint32_t f = sym_free(c, y, l, d, r);
-    uint32_t placed = 0;  // an item's subtree is at most (n - y0)! placements: rows left <= 11 fit
+    uint32_t placed = 0;  // an item's subtree is at most (n - y0)! placements: 11 rows left fit
     for (;;) {
         if (f == 0u) {  // row exhausted: back to the row above (within the item)
             if (y == y0) break;
@@ -270,8 +270,11 @@ extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, u
     if (n < 4) {  // too small for the bound rows: 1 (one class of one) for n = 1, none for 2 and 3
         host_unique = host_total = n == 1 ? 1u : 0u;
     } else {
-        // rows placed on the host: the device searches the last ~11 rows
-        h.depth = n <= 11 ? n : std::min(n - 3u, std::max(2u, n - 11u));
+        // Rows placed on the host: the device searches the last 11 rows, 12
+        // from N=18 on. Measured: one row less at N=16 left 106 workgroups
+        // (92 ms against 19); at N=18 one row more made 4.4 M items whose
+        // enumeration and upload took 155 of 278 ms (158 ms this way).
+        h.depth = n <= 11 ? n : std::min(n - 3u, std::max(2u, n < 18 ? n - 11u : n - 12u));
         const uint32_t last = n - 1u;
         for (uint32_t b1 = 2; b1 < last; ++b1) {  // corner: rows 0 and 1 fixed
             const SymCase c = sym_case(n, b1, 1u);
