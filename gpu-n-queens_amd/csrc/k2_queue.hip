@@ -17,11 +17,11 @@
 //    (one atomicAdd per wave refill for all idle lanes, ranks by mbcnt).
 //    When the queue is empty, idle lanes take work from busy lanes of the
 //    same wave: a donor hands over the bottom entry of its LDS stack (the
-//    shallowest open row, i.e. the largest untried subtree) and simply moves
-//    its stack base up; the receiver reads that 16-byte entry straight from
-//    the donor's LDS slot. Solves of different n mix freely: each lane holds
-//    its own column mask. Counts are folded per lane and flushed to per-solve
-//    u64 counters when the lane switches solve or exits.
+//    shallowest open row, i.e. the largest untried subtree), which the
+//    receiver reads straight from the donor's slot, and moves its other
+//    open rows down one slot. Solves of different n mix freely: each lane
+//    holds its own column mask. Counts are folded per lane and flushed to
+//    per-solve u64 counters when the lane switches solve or exits.
 //
 //  * Optional pass budget: when a wave has run `budget` passes, each lane
 //    with an unfinished piece parks it (current row + open rows) as one
@@ -56,6 +56,12 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Add one u64 per lane into arr[solve] (lanes with add == 0 contribute
@@ -412,51 +418,45 @@ __device__ __forceinline__ void switch_solve(const DrainArgs& a, bool sw, uint32
     if (sw) acc_solve = new_solve;
 }
 
-// LDS of one drain workgroup of T lanes (T = 64, 128 or 256):
-//   slot k of lane t       16 bytes in two 8-byte halves: {avail, ld} at
-//                          8 (t + 1) + 16T k and {rd, f} 8T bytes above. Each
-//                          half is a plane of T consecutive 8-byte words and
-//                          the slot stride 16T is a multiple of the 64 banks,
-//                          so one ds_read2st64_b64 / ds_write2st64_b64 moves a
-//                          lane's whole entry without bank conflicts, whatever
-//                          the stack depth of each lane;
-//   the inert entry        at 0 and 8T: address 0 is below every lane, 8T is
-//                          half 0 of slot 0 of lane T-1, which is reserved and
-//                          never holds work;
-//   after rows slots:      64 u32 per wave of donation scratch.
-// The inert entry {f=0, avail=1, ld=~0, rd=2} is a row that is never done:
-// placing its (empty) candidate gives avail 1 (no solution), no child cell
-// (so nothing is entered or counted) and leaves the entry as it was. A lane
-// whose stack runs empty pops it (the pop saturates at address 0) and from
-// then on runs harmless passes until the next refill; an idle lane is simply
-// one whose stack pointer is 0. The search loop therefore needs no per-lane
-// exit and no per-pass vote.
-__device__ __forceinline__ uint4 inert_entry() { return make_uint4(0u, 1u, 0xffffffffu, 2u); }
-
+// LDS of one drain workgroup of T lanes (T = 64, 128 or 256), rows of 16 T
+// bytes:
+//   row 0                  the donation scratch (64 u32 per wave); a lane
+//                          whose stack is empty reads its 16 bytes at 16 t
+//                          here when it pops, and ignores them;
+//   row j + 1              slot j of every lane: 16 bytes {f, avail, ld, rd}
+//                          at 16 t + 16 T (j + 1). A row is 16 T consecutive
+//                          bytes, so the ds_read_b128 / ds_write_b128 of a
+//                          wave reads or writes each 4-byte bank once per 16
+//                          lanes whatever the stack depth of each lane (the
+//                          row stride is a multiple of the 256 bytes the banks
+//                          span). One b128 read of a wave costs 4 LDS cycles
+//                          on gfx950 against 7.3 for the two b64 halves of
+//                          ds_read2st64_b64 (measured).
+// A lane's stack register is q = 16 t + 16 T k with k open rows. A push
+// stores at q + 16 T (the offset field of the instruction) and adds 16 T; a
+// pop subtracts 16 T and loads at q + 16 T. The subtraction borrows exactly
+// when the stack is empty (16 t < 16 T), so one v_sub_co_u32 both moves the
+// pointer and says that the piece is done; the load then reads 16 t (the
+// address sum wraps modulo 2^32), the lane's own slot of row 0.
 template <uint32_t T>
 struct Stack {
-    static constexpr uint32_t kHalf = 8u * T;      // bytes per half plane
-    static constexpr uint32_t kSlot = 2u * kHalf;  // bytes per slot (all lanes)
-    // byte address of slot 0 of lane t (half 0)
-    static __device__ __forceinline__ uint32_t base(uint32_t t) { return 8u * (t + 1u); }
-    static size_t bytes(uint32_t rows) { return (size_t)rows * kSlot + (size_t)T * sizeof(uint32_t); }
+    static constexpr uint32_t kRow = 16u * T;  // bytes from one slot of a lane to its next
+    static __device__ __forceinline__ uint32_t base(uint32_t t) { return 16u * t; }
+    static size_t bytes(uint32_t rows) { return (size_t)(rows + 1u) * kRow; }
 };
+static_assert(64u * sizeof(uint32_t) <= 16u * 64u, "a wave's donation scratch fits its lanes' row-0 slots");
 
 // Stack slots are addressed by their LDS byte offset; the dynamic LDS of the
 // drain is the only LDS of the kernel, so offset == address.
-typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef uint32_t lds_v4u_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) lds_v4u_t lds_v4u;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
-template <uint32_t T>
 __device__ __forceinline__ uint4 slot_load(uint32_t addr) {
-    const lds_u64* p = reinterpret_cast<const lds_u64*>((size_t)addr);
-    const uint64_t h0 = p[0], h1 = p[Stack<T>::kHalf / 8u];
-    return make_uint4((uint32_t)(h1 >> 32), (uint32_t)h0, (uint32_t)(h0 >> 32), (uint32_t)h1);
+    const lds_v4u_t v = *reinterpret_cast<const lds_v4u*>((size_t)addr);
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
-template <uint32_t T>
-__device__ __forceinline__ void slot_store(uint32_t addr, uint4 st) {
-    lds_u64* p = reinterpret_cast<lds_u64*>((size_t)addr);
-    p[0] = (uint64_t)st.y | (uint64_t)st.z << 32;
-    p[Stack<T>::kHalf / 8u] = (uint64_t)st.w | (uint64_t)st.x << 32;
+__device__ __forceinline__ void slot_store(uint32_t addr, uint4 s) {
+    *reinterpret_cast<lds_v4u*>((size_t)addr) = lds_v4u_t{s.x, s.y, s.z, s.w};
 }
 
 #ifndef NQM_PASS_UNROLL
@@ -466,91 +466,179 @@ constexpr uint32_t kPassUnroll = NQM_PASS_UNROLL;  // passes between two votes
 constexpr uint32_t kPhaseCap = 1u << 26;     // passes per phase: u32 counters cannot wrap
 constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per atomic
 
-// One search phase: every lane places one queen per pass (idle lanes spin on
-// the inert entry) until at most floor_live lanes hold a piece, or lim32
-// passes ran (checked every kPassUnroll passes). kWaveSols: psols is the
-// wave's solution count (same in every lane), else the lane's own. pnodes
-// gains the candidates of every row entered (placements made below it).
+// One search phase, entered by the lanes that hold a piece, written as one
+// asm block because it manages the execution mask itself: a lane whose piece
+// is done leaves the mask at once (it issues nothing more), and the phase
+// ends at a wave-uniform check every 8 passes, once at most floor_live
+// lanes are left or lim32 passes ran. In C++ the compiler's lowering of this
+// control flow spent 4-7 extra instructions per pass on copies and mask
+// merges. Here a pass is 13 VALU (2 for the pop, 11 for the step), 5 SALU
+// (with a wave-counted solution sum), one branch, one s_waitcnt and at most
+// one ds_read_b128 and one ds_write_b128 (measured at N=18: drain 15.4 ms
+// against 17.4 ms for the compiled loop with two b64 halves per slot).
+//
+// The drain is bound by the latency of this chain, not by LDS or VALU
+// throughput: a pop's b128 read returns after about 60 cycles unloaded and
+// 130 at 16 waves per CU, and the pass cannot place before it returns. So
+// the block keeps the chain short: the pop's mask comes from v_cmpx (no
+// SALU wait on a VALU result), the finished-piece bookkeeping sits in a
+// branch a wave rarely takes, and two register sets hold the current row
+// alternately (a pass reads its row from one and writes the child into the
+// other), so the push still has the row intact and the child's ops need not
+// wait for it.
+//
+// Registers: v[88:91] and v[92:95] = {f, avail, ld, rd} of the current row
+// (the b128 tuple of the pop and the push), alternating per pass; q = the
+// lane's stack register (layout above); alive = the lanes still placing. A
+// pass:
+//   pop    lanes whose row has no candidate left: q -= 16 T (the borrow marks
+//          a finished piece, which leaves alive; its placements, the pass
+//          index, are added to nodes) and load the slot at q + 16 T;
+//   step   b = f & -f; a solution when avail == b (every column taken);
+//          lanes with candidates left push {f ^ b, avail, ld, rd} at
+//          q + 16 T and move q up; the child into the other set:
+//          avail ^ b, (ld + b) << 1 (b is not in ld, so + is |),
+//          (rd | b) >> 1 and its free set avail & ~ld & ~rd in one v_bitop3
+//          (truth table 0x10).
+// Always entering the child keeps the step free of selects: a child
+// without a free cell is left again by the next pass's pop, which also
+// places the parent's next candidate, so every pass is one placement.
+// Loads complete inside the block (s_waitcnt before the first use), and
+// the block ends with the execution mask it started with. The carry-out of
+// v_subrev_co_u32 is zero for lanes off the mask, so it is the borrow mask
+// as it is.
+//
+// kWaveSols (every lane of one solve and weight): *psols and *pnodes are
+// the wave's solutions and placements, summed from lane masks in scalar
+// registers; else each lane's own. *ran = passes of the phase.
+#define NQM_K2_POP(U, CX, CT)                              \
+    "v_cmpx_eq_u32 vcc, 0, " CX "\n"                       \
+    "v_subrev_co_u32 %[q], vcc, %[row], %[q]\n"            \
+    "ds_read_b128 " CT ", %[q] offset:%[row]\n"            \
+    "s_cbranch_vccz .Lk2_nd" #U "_%=\n"                    \
+    "s_andn2_b64 %[alive], %[alive], vcc\n"                \
+    "s_bcnt1_i32_b64 %[t], vcc\n"                          \
+    "s_add_u32 %[t2], %[ran], " #U "\n"                    \
+    "s_mul_i32 %[t], %[t], %[t2]\n"                        \
+    "s_add_u32 %[nodes], %[nodes], %[t]\n"                 \
+    ".Lk2_nd" #U "_%=:\n"                                  \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    "s_waitcnt lgkmcnt(0)\n"
+#define NQM_K2_STEP(CX, CA, CL, CR, CT, NX, NA, NL, NR, SOLCOUNT) \
+    "v_sub_u32 %[n], 0, " CX "\n"                          \
+    "v_and_b32 %[b], " CX ", %[n]\n"                       \
+    "v_xor_b32 " NA ", " CA ", %[b]\n"                     \
+    "v_or_b32 " NR ", " CR ", %[b]\n"                      \
+    "v_add_lshl_u32 " NL ", %[b], " CL ", 1\n"             \
+    "v_cmp_eq_u32 %[sm], " CA ", %[b]\n"                   \
+    "v_cmpx_ne_u32 vcc, " CX ", %[b]\n"                    \
+    "v_xor_b32 " CX ", " CX ", %[b]\n"                     \
+    "ds_write_b128 %[q], " CT " offset:%[row]\n"           \
+    "v_add_u32 %[q], %[row], %[q]\n"                       \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    "v_lshrrev_b32 " NR ", 1, " NR "\n"                    \
+    SOLCOUNT                                               \
+    "v_bitop3_b32 " NX ", " NA ", " NL ", " NR " bitop3:0x10\n"
+#define NQM_K2_SOLS_WAVE                                   \
+    "s_bcnt1_i32_b64 %[t], %[sm]\n"                        \
+    "s_add_u32 %[sols], %[sols], %[t]\n"
+#define NQM_K2_SOLS_LANE                                   \
+    "v_addc_co_u32 %[sols], vcc, 0, %[sols], %[sm]\n"
+#define NQM_K2_PASS_A(U, SC) NQM_K2_POP(U, "v88", "v[88:91]") NQM_K2_STEP("v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", SC)
+#define NQM_K2_PASS_B(U, SC) NQM_K2_POP(U, "v92", "v[92:95]") NQM_K2_STEP("v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", SC)
+#define NQM_K2_PASS_WAVE(U) NQM_K2_PASS_##U(NQM_K2_SOLS_WAVE)
+#define NQM_K2_PASS_LANE(U) NQM_K2_PASS_##U(NQM_K2_SOLS_LANE) "v_add_u32 %[lnodes], 1, %[lnodes]\n"
+#define NQM_K2_PASS_0(SC) NQM_K2_PASS_A(0, SC)
+#define NQM_K2_PASS_1(SC) NQM_K2_PASS_B(1, SC)
+#define NQM_K2_PASS_2(SC) NQM_K2_PASS_A(2, SC)
+#define NQM_K2_PASS_3(SC) NQM_K2_PASS_B(3, SC)
+#define NQM_K2_PASS_4(SC) NQM_K2_PASS_A(4, SC)
+#define NQM_K2_PASS_5(SC) NQM_K2_PASS_B(5, SC)
+#define NQM_K2_PASS_6(SC) NQM_K2_PASS_A(6, SC)
+#define NQM_K2_PASS_7(SC) NQM_K2_PASS_B(7, SC)
+#define NQM_K2_PASS8(P) P(0) P(1) P(2) P(3) P(4) P(5) P(6) P(7)
+#define NQM_K2_PHASE(P, END)                               \
+    "s_mov_b64 %[live], exec\n"                            \
+    "s_mov_b64 %[alive], exec\n"                           \
+    "s_mov_b32 %[ran], 0\n"                                \
+    ".Lk2_pass_%=:\n" NQM_K2_PASS8(P)                      \
+    "s_add_u32 %[ran], %[ran], 8\n"                        \
+    "s_bcnt1_i32_b64 %[t], exec\n"                         \
+    "s_cmp_le_u32 %[t], %[floor]\n"                        \
+    "s_cbranch_scc1 .Lk2_end_%=\n"                         \
+    "s_cmp_lt_u32 %[ran], %[lim]\n"                        \
+    "s_cbranch_scc1 .Lk2_pass_%=\n"                        \
+    ".Lk2_end_%=:\n" END                                   \
+    "s_mov_b64 exec, %[live]\n"                            \
+    "s_waitcnt lgkmcnt(0)\n"
+static_assert(NQM_PASS_UNROLL == 8, "the asm phase votes every 8 passes");
+
+typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
+
 template <uint32_t T, bool kWaveSols>
-__device__ __forceinline__ void search_phase(uint4& st, uint32_t& top, uint32_t floor_live, uint32_t lim32,
+__device__ __forceinline__ void search_phase(uint4& st, uint32_t& q, uint32_t floor_live, uint32_t lim32,
                                              uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
-    constexpr uint32_t step = Stack<T>::kSlot;
-    // A row's candidates are counted at the start of the pass after it was
-    // entered, while st.x still holds them (a lane that pops has st.x == 0,
-    // so the pop can overwrite st.x in place). The row held at entry was
-    // counted already, the row held at exit not yet.
-    pnodes -= (uint32_t)__popc(st.x);
-    for (;;) {
-#pragma unroll
-        for (uint32_t u = 0; u < kPassUnroll; ++u) {
-            pnodes += (uint32_t)__popc(st.x);
-            // pins the count here: deferred, it would need a copy of st.x
-            asm volatile("" ::"v"(pnodes));
-            // a lane pops when its current row has no candidate left: the row
-            // it just entered had no free cell, or it returned to an
-            // exhausted row; it resumes the nearest open row
-            if (st.x == 0) {
-                top = __builtin_elementwise_sub_sat(top, step);
-                st = slot_load<T>(top);
-            }
-            const uint32_t b = st.x & (0u - st.x);
-            const uint32_t xr = st.x ^ b;  // the row's other candidates
-            const uint32_t a2 = st.y ^ b;
-            const uint32_t l2 = (st.z + b) << 1;  // b is not in ld: + is |
-            const uint32_t r2 = (st.w | b) >> 1;
-            // f2 = a2 & ~l2 & ~r2 in one op (truth table 0x10 over its inputs)
-            const uint32_t f2 = __builtin_amdgcn_bitop3_b32(a2, l2, r2, 0x10);
-            if (kWaveSols)
-                psols += (uint32_t)__popcll(__ballot(a2 == 0));
-            else
-                psols += (a2 == 0);
-            // Always enter the child: the parent waits on the stack if it has
-            // candidates left, and a child without a free cell is left again
-            // by the next pass's pop (which also places the parent's next
-            // candidate, so a pass is still one placement). Entering needs no
-            // select between parent and child state.
-            if (xr != 0) {
-                slot_store<T>(top, make_uint4(xr, st.y, st.z, st.w));
-                top += step;
-            }
-            st = make_uint4(f2, a2, l2, r2);
-        }
-        ran += kPassUnroll;
-        const uint32_t running = __popcll(__ballot(top != 0));
-        if (running <= floor_live || ran >= lim32) break;
+    v4u_t s4 = {st.x, st.y, st.z, st.w};
+    v4u_t s5;  // the other register set (scratch outside the block)
+    uint64_t live, alive, sm;
+    uint32_t b, n, t, t2;
+    uint32_t sols = 0, nodes = 0;
+    if (kWaveSols) {
+        asm volatile("s_mov_b32 %[sols], 0\n"
+                     "s_mov_b32 %[nodes], 0\n" NQM_K2_PHASE(NQM_K2_PASS_WAVE,
+                                                            "s_bcnt1_i32_b64 %[t], exec\n"
+                                                            "s_mul_i32 %[t], %[t], %[ran]\n"
+                                                            "s_add_u32 %[nodes], %[nodes], %[t]\n")
+                     : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), [q] "+v"(q), [b] "=&v"(b), [n] "=&v"(n),
+                       [live] "=&s"(live), [alive] "=&s"(alive), [sm] "=&s"(sm), [t] "=&s"(t), [t2] "=&s"(t2),
+                       [ran] "=&s"(ran), [sols] "=&s"(sols), [nodes] "=&s"(nodes)
+                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32)
+                     : "vcc", "scc", "memory");
+        psols = sols;
+        pnodes = nodes;
+    } else {
+        // the finished-piece sum in nodes is unused here: lnodes counts
+        // each lane's own placements
+        uint32_t lsols = psols, lnodes = pnodes;
+        asm volatile(NQM_K2_PHASE(NQM_K2_PASS_LANE, "")
+                     : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), [q] "+v"(q), [b] "=&v"(b), [n] "=&v"(n),
+                       [live] "=&s"(live), [alive] "=&s"(alive), [sm] "=&s"(sm), [t] "=&s"(t), [t2] "=&s"(t2),
+                       [ran] "=&s"(ran), [sols] "+v"(lsols), [nodes] "=&s"(nodes), [lnodes] "+v"(lnodes)
+                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32)
+                     : "vcc", "scc", "memory");
+        psols = lsols;
+        pnodes = lnodes;
     }
-    pnodes += (uint32_t)__popc(st.x);
+    (void)s5;
+    st = make_uint4(s4.x, s4.y, s4.z, s4.w);
 }
 
 template <uint32_t T>
 __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     extern __shared__ uint4 lds[];
-    constexpr uint32_t step = Stack<T>::kSlot;  // next slot of the same lane
+    constexpr uint32_t row = Stack<T>::kRow;  // next slot of the same lane
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    const uint32_t base = Stack<T>::base(tid);  // > 0: 0 means "no piece"
-    const bool reserved = tid == T - 1u;        // its slot 0 holds the inert entry
-    uint32_t* const wscratch =
-        reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + (size_t)a.stack_rows * step) + wave * 64u;
+    const uint32_t base = Stack<T>::base(tid);
+    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds) + wave * 64u;  // in row 0
     // the stack offsets below are LDS addresses: the dynamic LDS must start at 0
     if ((uint32_t)(size_t)(lds_u32*)lds != 0u) {
         if (tid == 0) atomicOr(a.err, 16u);
         return;
     }
-    if (tid == 0) slot_store<T>(0u, inert_entry());
-    __syncthreads();
     const uint32_t count = *a.item_count;
     const uint32_t waves = gridDim.x * (T / 64u);
 
     // lane state: st = {f, avail, ld, rd} of the current row; the open rows
-    // are the LDS slots [bot, top); slots in [base, bot) were donated and now
-    // hold the inert entry. top == 0: the lane holds no piece.
-    uint4 st = inert_entry();
+    // are slots 0..k-1, k = (q - base) / row (slot j at base + row (j + 1)).
+    // The lane holds a piece while st.x != 0 or q != base.
+    uint4 st = make_uint4(0u, 0u, 0u, 0u);
     uint32_t mask = 0, solve = 0, ws = 0;
-    uint32_t bot = 0, top = 0;
+    uint32_t q = base;
     uint32_t acc_solve = 0;
     uint64_t acc_sols = 0, acc_nodes = 0, exec_nodes = 0;
+    auto holds = [&]() { return st.x != 0u || q != base; };
 
     // wave-uniform queue state: items [lq, lq_end) are taken from the queue
     // and not yet started; rem is what the current segment held after the
@@ -569,8 +657,8 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     bool cont_open = cont_count != 0;
     uint64_t pass = 0;  // wave-uniform: passes run so far
 
-    // resume parked pieces exactly where they stopped (their nodes are
-    // counted already): current row in VGPRs, open rows to LDS
+    // resume parked pieces exactly where they stopped: current row in VGPRs,
+    // open rows to LDS
     auto resume_parked = [&](uint64_t& idle) {
         const uint32_t nidle = __popcll(idle);
         uint32_t cb = 0;
@@ -579,7 +667,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         const uint32_t cgot = cb >= cont_count ? 0u : min(nidle, cont_count - cb);
         if (cgot < nidle) cont_open = false;
         const uint32_t r = lane_rank(idle);
-        const bool take = top == 0 && !reserved && r < cgot;
+        const bool take = !holds() && r < cgot;
         const Cont* const c = a.cont_in + cb + r;
         const uint32_t csolve = take ? c->solve : 0u;
         switch_solve(a, take && csolve != acc_solve, csolve, acc_solve, acc_sols, acc_nodes, lane);
@@ -594,28 +682,22 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
             solve = csolve;
             ws = meta_wshift(cmeta);
             st = *reinterpret_cast<const uint4*>(c->st);
-            for (uint32_t j = 0; j < k; ++j) {
-                const uint4 e = *reinterpret_cast<const uint4*>(c->rows[j]);
-                slot_store<T>(base + j * step, make_uint4(e.w, e.x, e.y, e.z));
-            }
-            bot = base;
-            top = base + k * step;
+            for (uint32_t j = 0; j < k; ++j) slot_store(base + row * (j + 1u), *reinterpret_cast<const uint4*>(c->rows[j]));
+            q = base + k * row;
         }
-        idle = __ballot(top == 0 && !reserved);
+        idle = __ballot(!holds());
     };
 
     for (;;) {
         // ---- pass budget: park every unfinished piece, hand back every held
         // item, then leave
         if (a.budget && pass >= a.budget) {
-            // a piece is unfinished if its current row has a candidate or it
-            // has open rows ([bot, top); below bot are donated slots)
-            const bool park = top != 0 && (st.x != 0 || top > bot);
+            const bool park = holds();
             // once the queue and the parked pieces are used up, lanes would
             // idle next round: a piece with an open row parks its bottom row
             // (its largest subtree) as a piece of its own, so the next round
             // spreads the work again
-            const bool split = park && !queue_open && lq == lq_end && !cont_open && top > bot;
+            const bool split = park && !queue_open && lq == lq_end && !cont_open && q != base;
             const uint64_t pm = __ballot(park), sm = __ballot(split);
             const uint32_t np = (uint32_t)__popcll(pm) + (uint32_t)__popcll(sm);
             if (np) {
@@ -626,29 +708,28 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     if (lane == 0) atomicOr(a.err, 4u);
                 } else if (park) {
                     const uint32_t meta = item_meta(31u - __clz(mask + 1u), 0u, ws);
+                    uint32_t bot = base + row;  // slot 0
                     if (split) {
                         Cont* c = a.cont_out + o + (uint32_t)__popcll(pm) + lane_rank(sm);
                         c->solve = solve;
                         c->meta = meta;
                         c->k = 0;
                         c->pad = 0;
-                        *reinterpret_cast<uint4*>(c->st) = slot_load<T>(bot);
-                        bot += step;
+                        *reinterpret_cast<uint4*>(c->st) = slot_load(bot);
+                        bot += row;
                     }
                     Cont* c = a.cont_out + o + lane_rank(pm);
-                    const uint32_t k = top > bot ? (top - bot) / step : 0u;
+                    const uint32_t k = (q + row - bot) / row;
                     c->solve = solve;
                     c->meta = meta;
                     c->k = k;
                     c->pad = 0;
                     *reinterpret_cast<uint4*>(c->st) = st;
-                    for (uint32_t j = 0; j < k; ++j) {
-                        const uint4 e = slot_load<T>(bot + j * step);
-                        *reinterpret_cast<uint4*>(c->rows[j]) = make_uint4(e.y, e.z, e.w, e.x);
-                    }
+                    for (uint32_t j = 0; j < k; ++j) *reinterpret_cast<uint4*>(c->rows[j]) = slot_load(bot + j * row);
                 }
             }
-            top = 0;
+            st.x = 0;
+            q = base;
             if (lq < lq_end) {  // taken but never started: back to the queue
                 const uint32_t nloc = lq_end - lq;
                 uint32_t o = 0;
@@ -667,7 +748,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         // ---- refill idle lanes: parked pieces of earlier rounds, then held
         // items, then the global queue (one atomic per grab of up to kGrabMax
         // items), then in-wave donation
-        uint64_t idle = __ballot(top == 0 && !reserved);
+        uint64_t idle = __ballot(!holds());
         if (idle) {
             if (idle && cont_open && !a.items_first) resume_parked(idle);
             const uint32_t nidle = __popcll(idle);
@@ -694,7 +775,7 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                 }
                 const uint32_t r = lane_rank(idle);
                 uint32_t idx = 0xffffffffu;
-                if (top == 0 && !reserved) {
+                if (!holds()) {
                     if (r < take_local)
                         idx = lq + r;
                     else if (r - take_local < got)
@@ -709,22 +790,21 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     st = make_uint4(it.f, ~it.cols & mask, it.ld, it.rd);
                     solve = it.solve;
                     ws = meta_wshift(it.meta);
-                    bot = top = base;
-                    // the item's own candidates are placements of this piece
-                    const uint32_t pn = (uint32_t)__popc(it.f);
-                    exec_nodes += pn;
-                    acc_nodes += (uint64_t)pn << ws;
+                    q = base;
                 }
                 lq += take_local;
                 if (got) {  // the held range was used up: keep the rest of the grab
                     lq = g + min(got, nidle - take_local);
                     lq_end = g + got;
                 }
-                idle = __ballot(top == 0 && !reserved);
+                idle = __ballot(!holds());
             }
             if (idle && cont_open && a.items_first) resume_parked(idle);
             if (idle && a.donate) {
-                const bool is_donor = top > bot;
+                // a lane with an open row gives its bottom one (its largest
+                // untried subtree) to an idle lane, then moves its other open
+                // rows down one slot, so every stack still starts at its base
+                const bool is_donor = q != base;
                 const uint64_t donors = __ballot(is_donor);
                 const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
                 if (k) {
@@ -735,45 +815,42 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     const uint32_t ir = lane_rank(idle);
-                    const bool recv = top == 0 && !reserved && ir < k;
+                    const bool recv = !holds() && ir < k;
                     const uint32_t dl = recv ? wscratch[ir] : lane;
-                    const uint32_t d_bot = __shfl(bot, dl, 64);
                     const uint32_t d_mask = __shfl(mask, dl, 64);
                     const uint32_t d_solve = __shfl(solve, dl, 64);
                     const uint32_t d_ws = __shfl(ws, dl, 64);
                     switch_solve(a, recv && d_solve != acc_solve, d_solve, acc_solve, acc_sols, acc_nodes, lane);
                     if (recv) {
-                        // the donor's shallowest open row, straight from its LDS
-                        // slot; its candidates were counted when the donor
-                        // entered that row
-                        st = slot_load<T>(d_bot);
+                        // the donor's bottom open row, straight from its LDS slot
+                        st = slot_load(base - 16u * lane + 16u * dl + row);  // the donor's slot 0
                         mask = d_mask;
                         solve = d_solve;
                         ws = d_ws;
-                        bot = top = base;
+                        q = base;
                     }
                     // every receiver has read its entry before the donors overwrite it
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (gives) {
-                        slot_store<T>(bot, inert_entry());
-                        bot += step;
+                        for (uint32_t s = base + 2u * row; s <= q; s += row) slot_store(s - row, slot_load(s));
+                        q -= row;
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
             }
         }
-        const uint64_t live_mask = __ballot(top != 0);
+        const uint64_t live_mask = __ballot(holds());
         const bool have_items = queue_open || lq < lq_end || cont_open;
         if (live_mask == 0 && !have_items) break;
 
-        // ---- search: every lane places one queen per pass (the K1 step;
-        // idle lanes spin on the inert entry) until enough lanes idle to make
-        // a refill worth it (only while a refill can find work), every lane
-        // is idle, or the phase limit is reached
+        // ---- search: every lane with a piece places one queen per pass (the
+        // K1 step) until enough lanes idle to make a refill worth it (only
+        // while a refill can find work), every lane is idle, or the phase
+        // limit is reached
         const uint32_t nlive = __popcll(live_mask);
-        const bool can_refill = have_items || (a.donate && __ballot(top > bot) != 0);
+        const bool can_refill = have_items || (a.donate && __ballot(q != base) != 0);
         uint32_t floor_live = 0;
         uint64_t limit = a.budget ? a.budget - pass : (uint64_t)kPhaseCap;
         if (can_refill) {
@@ -785,19 +862,31 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         floor_live = __builtin_amdgcn_readfirstlane(floor_live);
         const uint32_t lim32 = __builtin_amdgcn_readfirstlane(
             (uint32_t)(limit < (uint64_t)kPhaseCap ? limit : (uint64_t)kPhaseCap));
-        // one (solve, weight) for every lane with a piece: count solutions per
-        // wave (a ballot and a scalar popcount) instead of per lane
+        // one (solve, weight) for every lane with a piece: the phase counts
+        // per wave (lane masks, scalar sums), else per lane
         const uint32_t lead = live_mask ? (uint32_t)__ffsll((long long)live_mask) - 1u : 0u;
         const uint32_t lead_solve = __shfl(solve, lead, 64);
         const uint32_t lead_ws = __shfl(ws, lead, 64);
-        const bool uniform = __ballot(top != 0 && (solve != lead_solve || ws != lead_ws)) == 0;
-        uint32_t psols = 0, pnodes = 0;  // this phase (<= kPhaseCap passes: no wrap)
-        uint32_t ran = 0;
-        if (uniform)
-            search_phase<T, true>(st, top, floor_live, lim32, psols, pnodes, ran);
-        else
-            search_phase<T, false>(st, top, floor_live, lim32, psols, pnodes, ran);
-        if (uniform) psols = lane == lead ? psols : 0u;  // the wave's count, once
+        const bool live = holds();
+        const bool uniform = __ballot(live && (solve != lead_solve || ws != lead_ws)) == 0;
+        uint32_t psols = 0, pnodes = 0, ran = 0;  // this phase (<= kPhaseCap passes: no wrap)
+        if (live) {
+            if (uniform)
+                search_phase<T, true>(st, q, floor_live, lim32, psols, pnodes, ran);
+            else
+                search_phase<T, false>(st, q, floor_live, lim32, psols, pnodes, ran);
+            // a lane whose piece ended borrowed below its stack; its last
+            // load (its own row-0 slot) is void
+            if ((int32_t)q < 0) {
+                q = base;
+                st.x = 0;
+            }
+        }
+        ran = wave_max_u32(ran);  // lanes without a piece did not enter
+        if (uniform) {  // the wave's counts, once
+            psols = lane == lead ? psols : 0u;
+            pnodes = lane == lead ? pnodes : 0u;
+        }
         exec_nodes += pnodes;
         acc_nodes += (uint64_t)pnodes << ws;
         acc_sols += (uint64_t)psols << ws;

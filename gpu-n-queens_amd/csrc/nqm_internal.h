@@ -102,8 +102,7 @@ hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_
 // ---- K2: persistent queue (k2_queue.hip) ----------------------------------
 struct Item;
 // A search piece parked at the end of a budgeted drain round: its current
-// row and its open rows (bottom first, in the LDS slot word order avail, ld,
-// rd, f). A later round resumes it exactly where it stopped, so its nodes
+// row and its open rows (bottom first), each as f, avail, ld, rd. A later round resumes it exactly where it stopped, so its nodes
 // are never counted twice and a round never multiplies the work items.
 constexpr uint32_t kContRows = 32;  // more than any stack needs (n <= 31)
 struct Cont {

@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libnqm.so")
+# NQM_LIB names another build of the same library (for comparing builds).
+LIB_PATH = os.environ.get("NQM_LIB") or os.path.join(_HERE, "libnqm.so")
 
 OK = 0
 EINVAL = 1
