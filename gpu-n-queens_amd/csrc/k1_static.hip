@@ -27,8 +27,10 @@
 //    places exactly one queen per pass, so its node count is the wave-uniform
 //    (scalar) pass counter at the moment it finishes.
 //  * Wave sum of u64 through cross-lane shuffles, LDS across the block, one
-//    store per block (the reference does 1024 same-address atomics,
-//    GPU_NQueens.cu:441-443).
+//    16-byte store per block (the reference does 1024 same-address atomics,
+//    GPU_NQueens.cu:441-443). Workgroups go round-robin to the 8 XCDs, each
+//    with its own L2, so the records are grouped per XCD (k1_part_slot): a
+//    cache line of records is written by one XCD only, not by eight.
 
 #include <type_traits>
 
@@ -71,8 +73,7 @@ __device__ __forceinline__ void block_sum2_u64(uint64_t& a, uint64_t& b, uint64_
 template <bool kWide>
 __global__ void __launch_bounds__(kK1MaxThreads)
 k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, uint32_t depth,
-                uint32_t flags, uint64_t* __restrict__ block_sols,
-                uint64_t* __restrict__ block_nodes, uint64_t* __restrict__ per_sols,
+                uint32_t flags, uint64_t* __restrict__ block_part, uint64_t* __restrict__ per_sols,
                 uint64_t* __restrict__ per_nodes, uint32_t* __restrict__ err) {
     extern __shared__ uint4 lds_stack[];  // [rows][blockDim.x]
     const uint32_t tid = threadIdx.x;
@@ -158,8 +159,9 @@ k1_count_kernel(const uint8_t* __restrict__ pool, uint64_t count, uint32_t n, ui
     __syncthreads();  // LDS stack is reused as reduction scratch
     block_sum2_u64(sols, nodes, reinterpret_cast<uint64_t*>(lds_stack));
     if (tid == 0) {
-        block_sols[blockIdx.x] = sols;
-        block_nodes[blockIdx.x] = nodes;
+        uint64_t* const r = block_part + 2u * k1_part_slot(blockIdx.x, gridDim.x);
+        r[0] = sols;
+        r[1] = nodes;
     }
 }
 
@@ -180,16 +182,13 @@ int k1_plan(const nqm_ctx* ctx, uint32_t n, uint32_t depth, uint64_t count, K1La
 
 hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_pool,
                      uint64_t count, uint32_t n, uint32_t depth, uint32_t flags,
-                     uint64_t* d_block_sols, uint64_t* d_block_nodes, uint64_t* d_per_sols,
-                     uint64_t* d_per_nodes, uint32_t* d_err) {
+                     uint64_t* d_block_part, uint64_t* d_per_sols, uint64_t* d_per_nodes, uint32_t* d_err) {
     if (n <= 18)
         hipLaunchKernelGGL(k1_count_kernel<false>, dim3(plan.blocks), dim3(plan.threads), plan.lds_bytes,
-                           stream, d_pool, count, n, depth, flags, d_block_sols, d_block_nodes,
-                           d_per_sols, d_per_nodes, d_err);
+                           stream, d_pool, count, n, depth, flags, d_block_part, d_per_sols, d_per_nodes, d_err);
     else
         hipLaunchKernelGGL(k1_count_kernel<true>, dim3(plan.blocks), dim3(plan.threads), plan.lds_bytes,
-                           stream, d_pool, count, n, depth, flags, d_block_sols, d_block_nodes,
-                           d_per_sols, d_per_nodes, d_err);
+                           stream, d_pool, count, n, depth, flags, d_block_part, d_per_sols, d_per_nodes, d_err);
     return hipGetLastError();
 }
 

@@ -82,41 +82,37 @@ static int count_device_impl(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uin
     K1Launch plan;
     int rc = k1_plan(ctx, n, depth, count, &plan);
     if (rc != NQM_OK) return fail(ctx, rc, "no launch shape for n=%u depth=%u", n, depth);
-    if ((rc = ensure_dev(ctx, ctx->part_sols, (size_t)plan.blocks * 8)) != NQM_OK) return rc;
-    if ((rc = ensure_dev(ctx, ctx->part_nodes, (size_t)plan.blocks * 8)) != NQM_OK) return rc;
+    const size_t recs = k1_part_records(plan.blocks);
+    if ((rc = ensure_dev(ctx, ctx->part, recs * 16)) != NQM_OK) return rc;
     if ((rc = ensure_dev(ctx, ctx->err, 8)) != NQM_OK) return rc;
-    if ((rc = ensure_host(ctx, ctx->h_part, (size_t)plan.blocks * 16 + 8)) != NQM_OK) return rc;
+    if ((rc = ensure_host(ctx, ctx->h_part, recs * 16 + 8)) != NQM_OK) return rc;
 
     NQM_HIP(ctx, hipMemsetAsync(ctx->err.ptr, 0, 4, ctx->stream));
     NQM_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
     hipError_t e = k1_launch(plan, ctx->stream, d_pool, count, n, depth, flags,
-                             static_cast<uint64_t*>(ctx->part_sols.ptr),
-                             static_cast<uint64_t*>(ctx->part_nodes.ptr), d_per_sols,
-                             d_per_nodes, static_cast<uint32_t*>(ctx->err.ptr));
+                             static_cast<uint64_t*>(ctx->part.ptr), d_per_sols, d_per_nodes,
+                             static_cast<uint32_t*>(ctx->err.ptr));
     if (e != hipSuccess)
         return fail(ctx, NQM_EDEVICE, "K1 launch failed: %s", hipGetErrorString(e));
     NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
 
     uint64_t* h = static_cast<uint64_t*>(ctx->h_part.ptr);
-    NQM_HIP(ctx, hipMemcpyAsync(h, ctx->part_sols.ptr, (size_t)plan.blocks * 8,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    NQM_HIP(ctx, hipMemcpyAsync(h + plan.blocks, ctx->part_nodes.ptr, (size_t)plan.blocks * 8,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    NQM_HIP(ctx, hipMemcpyAsync(h + 2 * (size_t)plan.blocks, ctx->err.ptr, 4,
-                                hipMemcpyDeviceToHost, ctx->stream));
+    NQM_HIP(ctx, hipMemcpyAsync(h, ctx->part.ptr, recs * 16, hipMemcpyDeviceToHost, ctx->stream));
+    NQM_HIP(ctx, hipMemcpyAsync(h + 2 * recs, ctx->err.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
     NQM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.f;
     NQM_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
 
     uint32_t err_flag = 0;
-    std::memcpy(&err_flag, h + 2 * (size_t)plan.blocks, 4);
+    std::memcpy(&err_flag, h + 2 * recs, 4);
     if (err_flag) return fail(ctx, NQM_EPREFIX, "pool holds an invalid prefix (n=%u depth=%u)", n, depth);
 
     // Host combine of per-block partials (GPU_NQueens.cu:162-172).
     uint64_t total = 0, nodes = 0;
-    for (uint32_t i = 0; i < plan.blocks; ++i) {
-        total += h[i];
-        nodes += h[plan.blocks + i];
+    for (uint32_t b = 0; b < plan.blocks; ++b) {
+        const uint64_t* r = h + 2 * (size_t)k1_part_slot(b, plan.blocks);
+        total += r[0];
+        nodes += r[1];
     }
     if (st) {
         st->total = total;
@@ -210,8 +206,7 @@ void nqm_destroy(nqm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_dev(ctx->pool);
-    free_dev(ctx->part_sols);
-    free_dev(ctx->part_nodes);
+    free_dev(ctx->part);
     free_dev(ctx->err);
     nqm::BatchWork& w = ctx->batch;
     nqm::drop_plan(w.plan);

@@ -60,7 +60,7 @@ struct nqm_ctx {
     hipDeviceProp_t prop{};
     uint32_t lds_per_block = 0;
     uint32_t lds_per_cu = 0;
-    nqm::DevBuf pool, part_sols, part_nodes, err;
+    nqm::DevBuf pool, part, err;
     nqm::BatchWork batch;
     nqm::HostBuf h_part;
     std::string last_error;
@@ -94,10 +94,16 @@ struct K1Launch {
     size_t lds_bytes = 0;
 };
 int k1_plan(const nqm_ctx* ctx, uint32_t n, uint32_t depth, uint64_t count, K1Launch* plan);
+// K1 writes one record {solutions, nodes} (two u64) per workgroup, grouped
+// per XCD: workgroup b (of nblocks) goes to record k1_part_slot(b, nblocks)
+// of k1_part_records(nblocks).
+__host__ __device__ inline uint32_t k1_part_slot(uint32_t b, uint32_t nblocks) {
+    return (b & 7u) * ((nblocks + 7u) / 8u) + (b >> 3);
+}
+__host__ __device__ inline uint32_t k1_part_records(uint32_t nblocks) { return 8u * ((nblocks + 7u) / 8u); }
 hipError_t k1_launch(const K1Launch& plan, hipStream_t stream, const uint8_t* d_pool,
                      uint64_t count, uint32_t n, uint32_t depth, uint32_t flags,
-                     uint64_t* d_block_sols, uint64_t* d_block_nodes, uint64_t* d_per_sols,
-                     uint64_t* d_per_nodes, uint32_t* d_err);
+                     uint64_t* d_block_part, uint64_t* d_per_sols, uint64_t* d_per_nodes, uint32_t* d_err);
 
 // ---- K2: persistent queue (k2_queue.hip) ----------------------------------
 struct Item;
