@@ -188,6 +188,10 @@ int nqm_create(int device, nqm_ctx** out) {
         delete ctx;
         return NQM_EDEVICE;
     }
+    if (nqm::static_scratch_init(device) != NQM_OK) {  // the allocation-free count's scratch
+        delete ctx;
+        return NQM_EDEVICE;
+    }
     ctx->lds_per_block = (uint32_t)ctx->prop.sharedMemPerBlock;
     ctx->lds_per_cu = (uint32_t)ctx->prop.maxSharedMemoryPerMultiProcessor;
     if (ctx->lds_per_block > 65536) ctx->lds_per_block = 65536;  // static launch limit

@@ -87,6 +87,10 @@ double now_ms();
 // Largest n this build counts (u32 bitsets), as deffinitions.cuh:116-118.
 constexpr uint32_t kMaxN = 31;
 
+// Looks up the device address of the allocation-free count's scratch array
+// for `device` (current), once per device (nqm_static.hip).
+int static_scratch_init(int device);
+
 // ---- K1: static pool (k1_static.hip) --------------------------------------
 struct K1Launch {
     uint32_t blocks = 0;

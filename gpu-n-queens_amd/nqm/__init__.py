@@ -142,6 +142,8 @@ _SIGNATURES = {
                         ctypes.c_uint32, U64P], ctypes.c_int),
     "nqm_count_pool_device": ([VP, ctypes.c_uint32, ctypes.c_uint32, VP, ctypes.c_uint64,
                                ctypes.c_uint32, VP, VP, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_count_pool_device_static": ([VP, ctypes.c_uint32, ctypes.c_uint32, VP, ctypes.c_uint64,
+                                      ctypes.c_uint32, VP], ctypes.c_int),
     "nqm_solver_driver": ([ctypes.c_uint32, VP, ctypes.c_uint32, ctypes.c_uint,
                            ctypes.POINTER(GpuConfig), ctypes.c_uint, U64P], ctypes.c_int),
     "nqm_state_size": ([ctypes.c_uint32], ctypes.c_size_t),
@@ -346,6 +348,17 @@ class Context:
                 ps.free()
             if pn:
                 pn.free()
+
+    def count_pool_device_static(self, n, depth, d_pool, count, d_result, flags=0):
+        """The allocation-free count (nqm_count_pool_device_static): enqueues
+        the count, whose three u64 results (solutions, nodes, invalid-prefix
+        flag) land in the device buffer d_result (a DeviceBuffer of at least
+        24 bytes, or a device pointer). Nothing is allocated and the call does
+        not wait; read d_result back (DeviceBuffer.to_host waits for the
+        context's stream)."""
+        self._check(self.lib.nqm_count_pool_device_static(
+            self.handle, n, depth, d_pool.ptr if isinstance(d_pool, DeviceBuffer) else d_pool, count, flags,
+            d_result.ptr if isinstance(d_result, DeviceBuffer) else d_result))
 
 
 def _solve_batch(ctx, solves, tune=None, raw=False, **tune_kw):

@@ -86,6 +86,19 @@ int nqm_count_pool_device(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uint8_
                           uint64_t count, uint32_t flags, uint64_t* d_per_sols,
                           uint64_t* d_per_nodes, nqm_stats* stats);
 
+/* The same count with no allocation of any kind (no hipMalloc/hipFree, no
+ * memset, no host synchronization): no counterpart in the reference; it is
+ * for callers that cannot allocate, such as code under HIP graph capture or
+ * a tight serving loop. Its scratch is a fixed 16 MiB per device, so pools
+ * of more than 2^26 = 67,108,864 prefixes are NQM_ERANGE. Results go to
+ * d_result (device, 3 x u64, written in the context's stream order):
+ * [0] = mirror-weighted solution count, [1] = nodes below the prefixes,
+ * [2] = nonzero if a prefix was invalid (the synchronous calls return
+ * NQM_EPREFIX instead). All calls on one device share the scratch: calls
+ * that overlap on one device race, so the caller serializes them. */
+int nqm_count_pool_device_static(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uint8_t* d_pool,
+                                 uint64_t count, uint32_t flags, uint64_t* d_result);
+
 /* The reference's entry point with its own argument layout: `states` is an
  * array of the reference's nq_state_t for board size n (n_queens.cuh:100-113:
  * u64 diagonal, u64 antidiagonal, u32 queens_in_columns, char curr_row,
