@@ -240,11 +240,53 @@ def _progress(label, every=100):
     return cb
 
 
+KNOWN = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
+         17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884}  # test.cu:18
+
+
+def serving_leg(ctx, requests=200, load=0.7):
+    """A bounded run of the serving workload for the default bench line: the
+    stream's first `requests` requests (N 10..20 ~ 1/N^2, 30% prefixed; they
+    include N=18, 19 and 20 boards). A closed loop over them gives the
+    capacity in requests/s; they are then offered as Poisson arrivals at
+    `load` of it. Every empty board is checked against the table, every
+    prefixed request against the same request solved alone."""
+    from nqm import serving, workloads
+    base = workloads.serving_stream(count=requests, rate=1.0)
+    srv = ctx.server()
+    try:
+        _, cwall = serving.replay(srv, [(0.0, n, p) for _, n, p in base])
+    finally:
+        srv.close()
+    capacity = requests / (cwall / 1e3)
+    reqs = workloads.serving_stream(count=requests, rate=load * capacity)
+    srv = ctx.server()
+    try:
+        res, wall = serving.replay(srv, reqs)
+        rs = srv.round_stats()
+    finally:
+        srv.close()
+    summ = serving.latency_summary(res, wall)
+    bad = []
+    for t, (_, n, p) in enumerate(reqs):
+        if not p:
+            if res[t][0] != KNOWN[n]:
+                bad.append(t)
+        else:
+            c, nd, _ = ctx.solve_batch([(n, tuple(p))])
+            if (int(c[0]), int(nd[0])) != (res[t][0], res[t][1]):
+                bad.append(t)
+    return {"requests": requests, "sizes": sorted({n for _, n, _ in reqs}),
+            "closed_loop_requests_per_s": capacity, "offered_per_s": load * capacity,
+            "p50_ms": summ["p50_ms"], "p99_ms": summ["p99_ms"], "max_ms": summ["max_ms"],
+            "solves_per_s": summ["solves_per_s"], "rounds": rs["rounds"],
+            "parked_max": rs["parked_max"], "lanes": rs["lanes"], "mismatches": bad}
+
+
 def workload_serving(ctx, nqm, args):
     import numpy as np
     from nqm import serving, workloads
-    known = {10: 724, 11: 2680, 12: 14200, 13: 73712, 14: 365596, 15: 2279184, 16: 14772512,
-             17: 95815104, 18: 666090624, 19: 4968057848, 20: 39029188884}  # test.cu:18
+    known = KNOWN
     # Capacity. The stream's work sits in its few large boards (about 5% are
     # N=20), so a sample's request rate says little about the whole stream's.
     # A closed loop over every 10th request gives work per second (executed
@@ -483,6 +525,10 @@ def main():
         w, st = min(runs, key=lambda x: x[0])
         line["symmetry_reduced"] = {"ms": w, "kernel_ms": st.kernel_ms, "unique": u, "total": t,
                                     "placements": st.nodes, "path": "K3 static kernel, first version"}
+    if args.variant == "all":
+        line["serving"] = serving_leg(ctx)
+        if line["serving"]["mismatches"]:
+            raise RuntimeError(f"serving mismatches: {line['serving']['mismatches']}")
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)

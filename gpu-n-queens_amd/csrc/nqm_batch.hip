@@ -221,6 +221,8 @@ int replay_plan(nqm_ctx* ctx, uint32_t ns, uint64_t* counts, uint64_t* nodes_out
         stats->launches = 1;
         stats->blocks = b.plan.blocks;
         stats->search_ms = (double)run_ms;  // the graph as a whole
+        stats->rounds = 1;
+        stats->lanes = b.plan.lanes;
     }
     return NQM_OK;
 }
@@ -601,6 +603,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     uint32_t launches = levels + split_levels + (uint32_t)goal_cap.size(), rounds = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    uint64_t round_parked_max = 0, round_handed_max = 0, round_room = 0;
 
     if (use_graph) {
         // One graph: clears, every expand level, the learned stage, the
@@ -660,6 +663,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         b.plan.host_nodes = h.nodes;
         b.plan.host_exec = h.exec_nodes;
         b.plan.blocks = blocks;
+        b.plan.lanes = (uint64_t)blocks * threads;
         graph = nullptr;
         gexec = nullptr;
     } else {
@@ -679,6 +683,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         const uint64_t spill_room = (uint64_t)blocks * (threads / 64u) * kGrabItems;
         const bool rounds_in_place = t.budget != 0;
         uint32_t* d_cc = nullptr;  // in count, head, out count of parked pieces
+        uint64_t parked_max = 0, handed_max = 0;  // per round
         uint32_t conts = 0;        // parked pieces waiting for the next round
         int ci = 0;                // b.cont[ci] holds them
         if (t.budget) {
@@ -734,6 +739,8 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             if (t.budget) NQM_HIP(ctx, hipMemcpyAsync(cc, d_cc, 12, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
             const uint32_t h0 = std::min(hd, qn);  // items [h0, qn) were never taken
+            parked_max = std::max<uint64_t>(parked_max, cc[2]);
+            handed_max = std::max<uint64_t>(handed_max, end > qn ? end - qn : 0u);
             queue += h0;
             qstart += h0;
             qn = end - h0;
@@ -749,6 +756,9 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
             if (rounds > 1000000u) return fail(ctx, NQM_ERANGE, "restart rounds do not converge");
         }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        round_parked_max = parked_max;
+        round_handed_max = handed_max;
+        round_room = spill_room;
     }
 
     std::vector<uint64_t> res(2 + 2 * (size_t)ns);
@@ -783,8 +793,12 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
         stats->launches = launches;
         stats->blocks = blocks;
         stats->search_ms = (double)search_ms;
+        stats->rounds = rounds;
+        stats->lanes = (uint64_t)blocks * threads;
+        stats->parked_max = round_parked_max;
+        stats->handed_back_max = round_handed_max;
+        stats->hand_back_room = round_room;
     }
-    (void)rounds;
     (void)split_levels;
     return NQM_OK;
 }

@@ -40,6 +40,7 @@ struct GraphPlan {
     std::vector<uint64_t> host_sols, host_nodes;  // placed on the host when it was built
     uint64_t host_exec = 0;
     uint32_t blocks = 0;
+    uint64_t lanes = 0;
 };
 struct BatchWork {
     // res: u64 executed nodes, u32 error bits, u32 pad, then per solve u64

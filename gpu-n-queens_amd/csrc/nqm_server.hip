@@ -109,6 +109,8 @@ struct nqm_server {
     std::chrono::steady_clock::time_point t0;
     uint32_t active = 0;
     uint64_t rounds_total = 0;
+    // per round: pieces parked, unstarted items handed back, and the bounds
+    uint64_t round_lanes = 0, parked_max = 0, handed_back_max = 0, hand_back_room = 0;
     ~nqm_server() {
         for (nqm::DevBuf* d : {&q[0], &q[1], &stage[0], &stage[1], &part, &counts, &head, &spill_count, &err,
                                &target, &sols, &nodes, &exec, &outst, &slot_list, &lvl_nodes, &cont[0], &cont[1],
@@ -473,6 +475,10 @@ int nqm_server_step(nqm_server* s, uint32_t* active_out) {
         NQM_HIP(ctx, hipStreamSynchronize(st));
         if (err) return fail(ctx, NQM_ERANGE, "server queue capacity exceeded (err=0x%x)", err);
         const uint32_t h0 = std::min(hd, s->qcount);  // items [h0, qcount) were never taken
+        s->round_lanes = std::max(s->round_lanes, lanes);
+        s->hand_back_room = std::max(s->hand_back_room, spill_bound);
+        s->parked_max = std::max<uint64_t>(s->parked_max, cc[2]);
+        s->handed_back_max = std::max<uint64_t>(s->handed_back_max, end > s->qcount ? end - s->qcount : 0u);
         if (end > s->qcount) {  // handed back after the old tail
             const uint64_t lo = s->qstart + s->qcount + s->shift, hi = s->qstart + end + s->shift;
             if (!s->handed_back.empty() && s->handed_back.back().second == lo)
@@ -554,6 +560,17 @@ int nqm_server_poll(nqm_server* s, nqm_result* out, uint32_t max, uint32_t* nout
     for (uint32_t i = 0; i < k; ++i) out[i] = s->done[i];
     s->done.erase(s->done.begin(), s->done.begin() + k);
     *nout = k;
+    return NQM_OK;
+}
+
+int nqm_server_round_stats(const nqm_server* s, nqm_stats* out) {
+    if (!s || !out) return NQM_EINVAL;
+    *out = nqm_stats{};
+    out->rounds = (uint32_t)std::min<uint64_t>(s->rounds_total, 0xffffffffu);
+    out->lanes = s->round_lanes;
+    out->parked_max = s->parked_max;
+    out->handed_back_max = s->handed_back_max;
+    out->hand_back_room = s->hand_back_room;
     return NQM_OK;
 }
 

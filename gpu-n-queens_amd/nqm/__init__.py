@@ -52,7 +52,9 @@ class Stats(ctypes.Structure):
     _fields_ = [("total", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
                 ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
                 ("launches", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
-                ("search_ms", ctypes.c_double)]
+                ("search_ms", ctypes.c_double), ("rounds", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("lanes", ctypes.c_uint64), ("parked_max", ctypes.c_uint64),
+                ("handed_back_max", ctypes.c_uint64), ("hand_back_room", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -191,6 +193,7 @@ _SIGNATURES = {
     "nqm_server_poll": ([VP, ctypes.POINTER(Result), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)],
                         ctypes.c_int),
     "nqm_server_info": ([VP, ctypes.POINTER(ctypes.c_uint32), U64P, U64P], ctypes.c_int),
+    "nqm_server_round_stats": ([VP, ctypes.POINTER(Stats)], ctypes.c_int),
     "nqm_server_clock_ms": ([VP], ctypes.c_double),
     "nqm_dev_alloc": ([VP, ctypes.c_size_t, ctypes.POINTER(VP)], ctypes.c_int),
     "nqm_dev_free": ([VP, VP], ctypes.c_int),
@@ -661,6 +664,13 @@ class Server:
         self.ctx._check(self.lib.nqm_server_info(self.handle, ctypes.byref(a), ctypes.byref(q),
                                                  ctypes.byref(r)))
         return {"active": a.value, "queued_items": q.value, "rounds": r.value}
+
+    def round_stats(self):
+        """Per-round maxima so far (nqm_server_round_stats): rounds, lanes,
+        parked_max, handed_back_max, hand_back_room."""
+        st = Stats()
+        self.ctx._check(self.lib.nqm_server_round_stats(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k in ("rounds", "lanes", "parked_max", "handed_back_max", "hand_back_room")}
 
 
 Context.server = lambda self, **cfg: Server(self, **cfg)

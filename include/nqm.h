@@ -53,6 +53,16 @@ typedef struct {
     uint32_t blocks;       /* workgroups of the (first) counting launch */
     double search_ms;      /* device time of the search launches alone (K1 count;
                               K2 drain rounds), without expansion or splitting */
+    /* Budgeted drain rounds (nqm_solve_batch with a budget; the server), for
+     * checking that rounds do not multiply the work: a round parks at most
+     * one piece per lane plus one split-off bottom row per lane, and hands
+     * back at most the queue items its waves held (waves x items per grab). */
+    uint32_t rounds;          /* drain launches */
+    uint32_t reserved;
+    uint64_t lanes;           /* lanes of a drain launch (the largest) */
+    uint64_t parked_max;      /* most pieces one round parked (<= 2 x lanes) */
+    uint64_t handed_back_max; /* most unstarted items one round handed back */
+    uint64_t hand_back_room;  /* what one round may hand back (<= this) */
 } nqm_stats;
 
 /* Create a context on HIP device `device` (its own stream, events and
@@ -315,6 +325,9 @@ int nqm_server_step(nqm_server* srv, uint32_t* active_out);
 int nqm_server_poll(nqm_server* srv, nqm_result* out, uint32_t max, uint32_t* nout);
 int nqm_server_info(const nqm_server* srv, uint32_t* active, uint64_t* queued_items,
                     uint64_t* rounds);
+/* The server's rounds so far in the round fields of nqm_stats (rounds, lanes,
+ * parked_max, handed_back_max, hand_back_room; the other fields are 0). */
+int nqm_server_round_stats(const nqm_server* srv, nqm_stats* out);
 /* Milliseconds since nqm_server_create (the clock of admit_ms / done_ms). */
 double nqm_server_clock_ms(const nqm_server* srv);
 

@@ -67,12 +67,19 @@ def test_restart_budget_is_exact(ctx, budget):
     assert np.array_equal(nd, np.array([r["nodes"] for r in g], np.uint64)), budget
     if budget <= 64:
         assert st.launches > 1  # restarts happened
+    # rounds never multiply the work: a round parks at most the piece and its
+    # split-off bottom row of each lane, and hands back at most what its
+    # waves held
+    assert st.parked_max <= 2 * st.lanes, (budget, st.parked_max, st.lanes)
+    assert st.handed_back_max <= st.hand_back_room, (budget, st.handed_back_max, st.hand_back_room)
 
 
 def test_restart_single_big_solve(ctx, known_totals, node_totals):
     c, nd, st = ctx.solve_batch([(16, ())], budget=2000, depth=4)
     assert int(c[0]) == known_totals[16] and int(nd[0]) == node_totals[16]
     assert st.launches > 2
+    assert st.rounds > 2 and st.parked_max <= 2 * st.lanes
+    assert st.handed_back_max <= st.hand_back_room
 
 
 def test_batch_alone_vs_in_mix(ctx):
