@@ -358,12 +358,41 @@ def workload_serving(ctx, nqm, args):
 
 
 def cpu_baseline(seconds):
-    """The oracle (C port of the reference's counting loop) on a bounded,
-    evenly strided sample of the N=18 lock-8 half pool, all host threads."""
+    """The oracle (C port of the reference's counting loop) on the host: the
+    whole C1 (N=12, all 756 depth-3 prefixes) and C2 (N=16, all 141,812
+    depth-5 prefixes) pools, each the median of 5 timed runs after one
+    warm-up on all host threads, and on one thread (C1 likewise, C2 once);
+    and a bounded, evenly strided
+    sample of the N=18 lock-8 half pool on all host threads (its rate is
+    `value`, with the extrapolated full N=18 time)."""
+    import statistics
     import numpy as np
     import oracle
-    pool = oracle.pool_generate(N, 8, mirror=True)
     threads = os.cpu_count() or 1
+
+    def timed(n, depth, pool, mirror, th, runs, warm=True):
+        if warm:
+            oracle.count_pool(n, depth, pool, mirror=mirror, threads=th)  # warm-up
+        walls = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            total, nodes = oracle.count_pool(n, depth, pool, mirror=mirror, threads=th)
+            walls.append(time.perf_counter() - t0)
+        w = statistics.median(walls)
+        return {"total": total, "nodes": nodes, "median_s": w, "runs": runs, "threads": th,
+                "nodes_per_s": nodes / w}
+
+    out = {}
+    for name, n, depth, expect in (("c1", 12, 3, 14200), ("c2", 16, 5, 14772512)):
+        pool = oracle.pool_generate(n, depth, mirror=False)
+        r = timed(n, depth, pool, False, threads, 5)
+        # one thread: C2 once (about 10 s on a server core), after the
+        # all-thread runs have warmed the pool
+        r1 = timed(n, depth, pool, False, 1, 5 if name == "c1" else 1, warm=name == "c1")
+        if r["total"] != expect or r1["total"] != expect:
+            raise RuntimeError(f"CPU baseline {name}: {r['total']} / {r1['total']} != {expect}")
+        out[name] = {"n": n, "depth": depth, "prefixes": int(pool.shape[0]), "all_threads": r, "one_thread": r1}
+    pool = oracle.pool_generate(N, 8, mirror=True)
     stride = 4096
     while True:
         sample = np.ascontiguousarray(pool[::stride])
@@ -382,13 +411,14 @@ def cpu_baseline(seconds):
     t0 = time.perf_counter()
     _, nodes1 = oracle.count_pool(N, 8, sample1, mirror=True, threads=1)
     dt1 = time.perf_counter() - t0
+    out["n18"] = {"sample": f"every {stride}th prefix of the N=18 lock-8 half pool "
+                            f"({sample.shape[0]} prefixes, {nodes} nodes, {dt:.1f} s)",
+                  "nodes_per_s": rate, "extrapolated_wall_s": HALF_NODES / rate,
+                  "one_thread": {"nodes_per_s": nodes1 / max(dt1, 1e-9),
+                                 "sample": f"every {stride1}th prefix ({nodes1} nodes, {dt1:.1f} s)"}}
     return {"value": rate, "unit": "nodes/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th prefix of the N=18 lock-8 half pool "
-                      f"({sample.shape[0]} prefixes, {nodes} nodes, {dt:.1f} s)",
-            "extrapolated_wall_s_n18": HALF_NODES / rate,
-            "one_thread": {"value": nodes1 / max(dt1, 1e-9), "unit": "nodes/s",
-                           "sample": f"every {stride1}th prefix ({nodes1} nodes, {dt1:.1f} s)"},
-            "host": host_cpu()}
+            "sample": out["n18"]["sample"], "extrapolated_wall_s_n18": HALF_NODES / rate,
+            "one_thread": out["n18"]["one_thread"], **out, "host": host_cpu()}
 
 
 def host_cpu():

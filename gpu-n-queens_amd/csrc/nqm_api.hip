@@ -269,13 +269,6 @@ int nqm_count_pool(nqm_ctx* ctx, uint32_t n, uint32_t depth, const uint8_t* pool
     return rc;
 }
 
-size_t nqm_state_size(uint32_t n) {
-    // n_queens.cuh:100-113: diagonals_t (16) + u32 (4) + char (1) + u8[n],
-    // aligned to the 8-byte alignment of its u64 members.
-    if (n < 1 || n > kMaxN) return 0;
-    return (size_t)((21 + n + 7) & ~7u);
-}
-
 int nqm_solver_driver(uint32_t n, const void* states, uint32_t state_cnt, unsigned locked_row_end,
                       const nqm_gpu_config* configs, unsigned config_cnt, uint64_t* total_out) {
     if (!states || !state_cnt || !configs || !config_cnt || !total_out) return NQM_EINVAL;

@@ -1,15 +1,18 @@
 // nqm_files.hip -- .nqsf pool files and the reference's nq_state_t layout.
 //
-// Host-only code. The file layout and the state layout are the reference's
-// (nq_utils.cu:127-259, n_queens.cuh:100-113), so pools move between the two
-// programs unchanged; the code is written for this library: bounded reads,
-// status codes instead of asserts, no N fixed at compile time.
+// Host-only code with no HIP dependency (it also builds with g++ for the
+// host sanitizer run of the parser). The file layout and the state layout
+// are the reference's (nq_utils.cu:127-259, n_queens.cuh:100-113), so pools
+// move between the two programs unchanged; the code is written for this
+// library: bounded reads, status codes instead of asserts, no N fixed at
+// compile time.
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <vector>
 
-#include "nqm_internal.h"
+#include "../../include/nqm.h"
+#include "nqm_limits.h"
 
 namespace {
 
@@ -48,6 +51,13 @@ void make_state(uint32_t n, uint32_t lock, const uint8_t* pre, uint8_t* st) {
 }  // namespace
 
 extern "C" {
+
+size_t nqm_state_size(uint32_t n) {
+    // n_queens.cuh:100-113: diagonals_t (16) + u32 (4) + char (1) + u8[n],
+    // aligned to the 8-byte alignment of its u64 members.
+    if (n < 1 || n > nqm::kMaxN) return 0;
+    return (size_t)((21 + n + 7) & ~7u);
+}
 
 void nqm_free(void* p) { std::free(p); }
 

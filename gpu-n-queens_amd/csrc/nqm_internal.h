@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/nqm.h"
+#include "nqm_limits.h"
 
 namespace nqm {
 
@@ -85,8 +86,6 @@ int ensure_dev(nqm_ctx* ctx, DevBuf& b, size_t bytes);
 int ensure_host(nqm_ctx* ctx, HostBuf& b, size_t bytes);
 double now_ms();
 
-// Largest n this build counts (u32 bitsets), as deffinitions.cuh:116-118.
-constexpr uint32_t kMaxN = 31;
 
 // Looks up the device address of the allocation-free count's scratch array
 // for `device` (current), once per device (nqm_static.hip).
