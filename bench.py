@@ -174,8 +174,10 @@ def workload_batch(ctx, nqm, args):
         ms = ms[args.warmup:]
         rows[name] = {"kernel_ms": sorted(ms)[len(ms) // 2], "total": int(c.sum()),
                       "nodes_executed": st.nodes, "launches": st.launches}
+    # the full grid of the batched config: prefix depth 3-7, waves/CU 4-16,
+    # LDS stack depth (rows per lane; the target row is raised to fit)
     best, trials = autotune.sweep(ctx, solves, depths=(3, 4, 5, 6, 7), waves=(4, 8, 12, 16),
-                                  repeats=2)
+                                  stack_rows=(6, 8, 10, 12), repeats=2)
     autotune.store(ctx, "mixed512", best, trials)
     b = rows["auto"]
     return {"metric": "512 mixed solves in one persistent launch: solves/s",

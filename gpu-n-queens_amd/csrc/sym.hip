@@ -16,6 +16,7 @@
 // LDS for the rotation check. Totals equal the reference's counts; the node
 // numbers are the reduced search's own.
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "nqm_internal.h"
@@ -110,6 +111,7 @@ struct SymItem {
 static_assert(sizeof(SymItem) == 48, "item layout");
 
 constexpr uint32_t kSymLanes = 64;  // one wave per workgroup
+constexpr size_t kSymMaxItems = (size_t)1 << 26;  // 3 GiB of host items; N=18 makes 0.9 M
 
 // LDS per workgroup: the stack, [row][lane] x 16 B {f, l, d, r} for rows
 // y0..last-1, then the board, [row][lane] x 1 B (column) for rows 0..31.
@@ -212,6 +214,8 @@ k3_sym_kernel(const SymItem* __restrict__ items, uint32_t count, uint32_t rows, 
 // host (small boards) are counted here.
 struct SymHost {
     uint32_t depth;
+    size_t max_items;       // the enumeration stops (overflow) past this many items
+    bool overflow = false;
     std::vector<SymItem> items;
     uint64_t c2 = 0, c4 = 0, c8 = 0, nodes = 0;
     uint8_t col[32];
@@ -223,7 +227,12 @@ struct HostBoard {
 };
 
 void sym_host_rec(SymHost& h, const SymCase& c, uint32_t y, uint32_t l, uint32_t d, uint32_t r) {
+    if (h.overflow) return;
     if (y == h.depth && y < c.last) {
+        if (h.items.size() >= h.max_items) {
+            h.overflow = true;
+            return;
+        }
         SymItem it{};
         it.l = l;
         it.d = d;
@@ -263,7 +272,8 @@ void sym_host_rec(SymHost& h, const SymCase& c, uint32_t y, uint32_t l, uint32_t
 using namespace nqm;
 
 extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
-    if (!ctx || n < 1 || n > kMaxN) return NQM_EINVAL;
+    if (!ctx) return NQM_EINVAL;
+    if (n < 1 || n > kMaxN) return fail(ctx, NQM_EINVAL, "n=%u outside 1..%u", n, kMaxN);
     const double t0 = now_ms();
     uint64_t host_unique = 0, host_total = 0;
     SymHost h;
@@ -275,22 +285,32 @@ extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, u
         // (92 ms against 19); at N=18 one row more made 4.4 M items whose
         // enumeration and upload took 155 of 278 ms (158 ms this way).
         h.depth = n <= 11 ? n : std::min(n - 3u, std::max(2u, n < 18 ? n - 11u : n - 12u));
+        // one thread per item, 64 per workgroup: the grid and the per-block
+        // records stay within u32 and the host buffer within a few GiB
+        h.max_items = kSymMaxItems;
         const uint32_t last = n - 1u;
-        for (uint32_t b1 = 2; b1 < last; ++b1) {  // corner: rows 0 and 1 fixed
-            const SymCase c = sym_case(n, b1, 1u);
-            const uint32_t bit = 1u << b1;
-            h.col[0] = 0;
-            h.col[1] = (uint8_t)b1;
-            h.nodes += 2;
-            sym_host_rec(h, c, 2, (2u | bit) << 1, 1u | bit, bit >> 1);
+        try {
+            for (uint32_t b1 = 2; b1 < last; ++b1) {  // corner: rows 0 and 1 fixed
+                const SymCase c = sym_case(n, b1, 1u);
+                const uint32_t bit = 1u << b1;
+                h.col[0] = 0;
+                h.col[1] = (uint8_t)b1;
+                h.nodes += 2;
+                sym_host_rec(h, c, 2, (2u | bit) << 1, 1u | bit, bit >> 1);
+            }
+            for (uint32_t b1 = 1; b1 < last - b1; ++b1) {  // non-corner: row 0 fixed
+                const SymCase c = sym_case(n, b1, 0u);
+                const uint32_t bit = 1u << b1;
+                h.col[0] = (uint8_t)b1;
+                h.nodes += 1;
+                sym_host_rec(h, c, 1, bit << 1, bit, bit >> 1);
+            }
+        } catch (const std::bad_alloc&) {
+            return fail(ctx, NQM_ENOMEM, "host items of the symmetric count (n=%u)", n);
         }
-        for (uint32_t b1 = 1; b1 < last - b1; ++b1) {  // non-corner: row 0 fixed
-            const SymCase c = sym_case(n, b1, 0u);
-            const uint32_t bit = 1u << b1;
-            h.col[0] = (uint8_t)b1;
-            h.nodes += 1;
-            sym_host_rec(h, c, 1, bit << 1, bit, bit >> 1);
-        }
+        if (h.overflow)
+            return fail(ctx, NQM_ERANGE, "n=%u: the symmetric count's host enumeration exceeds %zu items", n,
+                        kSymMaxItems);
     }
     uint64_t c2 = h.c2, c4 = h.c4, c8 = h.c8, nodes = h.nodes;
     float ms = 0.f;

@@ -2,7 +2,8 @@
 counts, then use it to split and order the work queue of large solves.
 
 Training data: every prefix (all row-0 columns) of N=13..16 at depths 4..6
-(2,027,144 samples). The pools come from the device generator; each label is
+(2,026,336 samples: the generator drops a prefix whose next row has no free
+cell). The pools come from the device generator; each label is
 log2(1 + nodes below the prefix) as K1 counts it (per-prefix output).
 Everything runs through the C ABI; nothing here counts on the CPU.
 """
