@@ -1,0 +1,1002 @@
+// k4_sym.hip -- K4: the symmetry-reduced count on a persistent drain.
+//
+// The tree is the one of sym_rules.h (corner / non-corner method), searched
+// the way K2 searches the plain tree: a queue of open nodes, lanes that draw
+// from it, an LDS stack per lane and intra-wave donation. Three steps:
+//
+//  * items: the host places the first rows, then k4_expand grows them level
+//    by level on the device to row R = ceil(n/2) - 1 (so that row 0's side
+//    rule, rows < b1 avoid the side columns, is done before the drain). A
+//    corner item whose rule "rows < b1 avoid column 1" reaches below R is
+//    grown to row b1 instead, so the drain never sees that rule.
+//
+//  * k4_drain: K2's pass plus the two row rules left below R, as selects:
+//      row b2 (non-corner): with one side column open, only it; with both
+//        open, nothing (the exact rule of sym_free, so the node numbers are
+//        the oracle's);
+//      last row (non-corner): no column within b1 of a corner.
+//    A lane does not know its row; a marker bit in the high, unused bits of
+//    its diagonals tells it. The ld marker sits 31 - (b2 - row) bits up, so
+//    it reaches bit 31 exactly when the child is at row b2 (and is shifted
+//    out after it); the rd marker sits n + (last - row) bits up and reaches
+//    bit n at the last row (rd only moves down, and never below n). Both ride
+//    along in the stack slots and in donated rows like the real bits. The
+//    item's ld is cut to n bits, so real diagonal bits stay below the markers
+//    for n <= 21.
+//    A full board whose class may have another representative (a tie of the
+//    top queen's corner distance: the last queen on column b2, the column-0
+//    queen on row b2 -- which puts bit b1 into the last row's ld -- or row
+//    b1's queen on column n-1) is not counted in the drain: the lane writes
+//    {item, ld of the last row} into its wave's record chunk. Every other
+//    full board is its class's only representative and counts 8.
+//
+//  * k4_ties: one thread per record searches the item again, keeping only
+//    the columns whose diagonal bit is in the recorded ld, and runs the
+//    rotation check (sym_check) on the boards it finds. Several boards of one
+//    item can share an ld, and each of them wrote the same record: a hash
+//    set keeps the first copy, which counts them all.
+//
+// Unique classes = drain leaves + tie classes; total = 8 x drain leaves +
+// tie weights; both equal K3's and the oracle's, and so do the nodes.
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "nqm_internal.h"
+#include "sym_rules.h"
+
+namespace nqm {
+namespace {
+
+// An open node of the symmetric search at row `row`.
+struct alignas(16) SymQ {
+    uint32_t f;       // candidates of `row` under the case's rules (never 0 in a queue)
+    uint32_t cols;    // columns of rows 0..row-1
+    uint32_t ld, rd;  // relative diagonals at `row`; ld cut to n bits
+    uint32_t meta;    // n | row << 8 | b1 << 16 | corner << 24 | top at b1 << 25
+    uint32_t pad;
+    uint32_t p0, p1;  // columns of rows 0..11, 5 bits each (row r at bit 5 r of p1:p0)
+};
+static_assert(sizeof(SymQ) == 32, "two 16-byte loads");
+
+__host__ __device__ inline uint32_t symq_meta(uint32_t n, uint32_t row, uint32_t b1, uint32_t corner,
+                                              uint32_t top_b1) {
+    return (n & 31u) | ((row & 31u) << 8) | ((b1 & 31u) << 16) | ((corner & 1u) << 24) | ((top_b1 & 1u) << 25);
+}
+__host__ __device__ inline uint32_t symq_n(uint32_t m) { return m & 31u; }
+__host__ __device__ inline uint32_t symq_row(uint32_t m) { return (m >> 8) & 31u; }
+__host__ __device__ inline uint32_t symq_b1(uint32_t m) { return (m >> 16) & 31u; }
+__host__ __device__ inline uint32_t symq_corner(uint32_t m) { return (m >> 24) & 1u; }
+__host__ __device__ inline uint32_t symq_top_b1(uint32_t m) { return (m >> 25) & 1u; }
+
+constexpr uint32_t kPrefixRows = 12;  // rows whose columns an item keeps
+
+// The row an item is grown to: R, or b1 for a corner item whose column-1
+// rule reaches below R.
+__host__ __device__ inline uint32_t symq_target(uint32_t meta, uint32_t R) {
+    return (symq_corner(meta) && symq_b1(meta) > R) ? symq_b1(meta) : R;
+}
+
+// The child of item `it` (at row y) for column bit b; f = 0 if it has no
+// candidate.
+__host__ __device__ inline SymQ symq_child(const SymQ& it, const SymCase& c, uint32_t b) {
+    const uint32_t y = symq_row(it.meta);
+    SymQ ch;
+    ch.cols = it.cols | b;
+    ch.ld = ((it.ld | b) << 1) & c.mask;
+    ch.rd = (it.rd | b) >> 1;
+    ch.f = sym_free(c, y + 1u, ch.ld, ch.cols, ch.rd);
+    const uint32_t col = (uint32_t)__builtin_ctz(b);
+    uint64_t p = ((uint64_t)it.p1 << 32) | it.p0;
+    if (y < kPrefixRows) p |= (uint64_t)col << (5u * y);
+    ch.p0 = (uint32_t)p;
+    ch.p1 = (uint32_t)(p >> 32);
+    const uint32_t top_b1 = symq_top_b1(it.meta) | (!c.corner && y == c.b1 && b == c.top ? 1u : 0u);
+    ch.meta = symq_meta(c.n, y + 1u, c.b1, c.corner, top_b1);
+    ch.pad = 0;
+    return ch;
+}
+
+// ---------------------------------------------------------------------------
+// expansion: one level of children for every pending item
+// ---------------------------------------------------------------------------
+// A child at its target row goes to the final queue, one still below it to
+// the next level. Outputs are reserved with one atomic per wave; the order
+// within a level does not matter (every item is an independent subtree).
+// Counts: cnt[0] final, cnt[1] next level.
+__device__ __forceinline__ uint32_t k4_incl_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(256)
+k4_expand_count_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, uint32_t* __restrict__ cnt) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t fin = 0, nxt = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const SymQ it = in[i];
+        const SymCase c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
+        const uint32_t tgt = symq_target(it.meta, R);
+        for (uint32_t f = it.f; f; f &= f - 1u) {
+            const SymQ ch = symq_child(it, c, f & (0u - f));
+            if (ch.f) {
+                if (symq_row(ch.meta) >= tgt)
+                    ++fin;
+                else
+                    ++nxt;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        fin += __shfl_xor(fin, o, 64);
+        nxt += __shfl_xor(nxt, o, 64);
+    }
+    if (lane == 0) {
+        if (fin) atomicAdd(&cnt[0], fin);
+        if (nxt) atomicAdd(&cnt[1], nxt);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k4_expand_write_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, SymQ* __restrict__ fin_q,
+                       uint32_t fin_cap, SymQ* __restrict__ next_q, uint32_t next_cap, uint32_t* __restrict__ cnt,
+                       unsigned long long* __restrict__ nodes, uint32_t* __restrict__ err) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t placed = 0;
+    // block-uniform trip count: every lane takes part in the scans
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < count; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        SymQ it{};
+        uint32_t nf = 0, nn = 0;
+        SymCase c{};
+        uint32_t tgt = 0;
+        if (i < count) {
+            it = in[i];
+            c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
+            tgt = symq_target(it.meta, R);
+            placed += (uint32_t)__popc(it.f);
+            for (uint32_t f = it.f; f; f &= f - 1u) {
+                const SymQ ch = symq_child(it, c, f & (0u - f));
+                if (ch.f) {
+                    if (symq_row(ch.meta) >= tgt)
+                        ++nf;
+                    else
+                        ++nn;
+                }
+            }
+        }
+        const uint32_t sf = k4_incl_scan(nf, lane), sn = k4_incl_scan(nn, lane);
+        uint32_t bf = 0, bn = 0;
+        if (lane == 63u) {
+            if (sf) bf = atomicAdd(&cnt[2], sf);
+            if (sn) bn = atomicAdd(&cnt[3], sn);
+        }
+        bf = __shfl(bf, 63, 64) + sf - nf;
+        bn = __shfl(bn, 63, 64) + sn - nn;
+        if (nf | nn) {
+            for (uint32_t f = it.f; f; f &= f - 1u) {
+                const SymQ ch = symq_child(it, c, f & (0u - f));
+                if (!ch.f) continue;
+                if (symq_row(ch.meta) >= tgt) {
+                    if (bf < fin_cap)
+                        fin_q[bf] = ch;
+                    else
+                        atomicOr(err, 1u);
+                    ++bf;
+                } else {
+                    if (bn < next_cap)
+                        next_q[bn] = ch;
+                    else
+                        atomicOr(err, 1u);
+                    ++bn;
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) placed += __shfl_xor(placed, o, 64);
+    if (lane == 0 && placed) atomicAdd(nodes, (unsigned long long)placed);
+}
+
+// ---------------------------------------------------------------------------
+// drain
+// ---------------------------------------------------------------------------
+constexpr uint32_t kK4Threads = 256;  // 4 waves per workgroup
+constexpr uint32_t kRecChunk = 4096;  // records per chunk of a wave
+// A phase ends at a vote with at most this many records in the chunk: the 8
+// passes to the next vote add at most 8 x 64.
+constexpr uint32_t kRecLimit = kRecChunk - 8u * 64u;
+constexpr uint32_t kK4Grab = 128;  // queue items one wave takes per atomic at most
+constexpr uint32_t kK4MaxRows = 9;  // rows below an item (last - R, k4_supports)
+
+struct K4Args {
+    const SymQ* items;
+    uint32_t count;
+    uint32_t nseg;
+    uint32_t* head;       // nseg heads, kHeadStride u32 apart
+    uint32_t n, last, mask, side;
+    uint32_t refill_idle;
+    uint2* recs;          // record chunks (kRecChunk records each); chunk nchunks is the overflow sink
+    uint32_t* rec_counts; // records in each chunk, written when the wave closes it
+    uint32_t* chunk_head; // chunks handed out
+    uint32_t* rec_total;  // records written
+    uint32_t nchunks;
+    uint64_t* wave_part;  // per wave {leaves counted 8, executed placements}
+    uint32_t* err;
+};
+
+// LDS rows as in K2's drain: row 0 = donation scratch and the target of an
+// empty stack's pop; slot j of lane t at 16 t + 16 T (j + 1).
+constexpr uint32_t kK4Row = 16u * kK4Threads;
+
+typedef uint32_t k4_v4u_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) k4_v4u_t k4_lds_v4u;
+typedef __attribute__((address_space(3))) uint32_t k4_lds_u32;
+__device__ __forceinline__ uint4 k4_slot_load(uint32_t addr) {
+    const k4_v4u_t v = *reinterpret_cast<const k4_lds_v4u*>((size_t)addr);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void k4_slot_store(uint32_t addr, uint4 s) {
+    *reinterpret_cast<k4_lds_v4u*>((size_t)addr) = k4_v4u_t{s.x, s.y, s.z, s.w};
+}
+
+__device__ __forceinline__ uint32_t k4_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+__device__ __forceinline__ uint64_t k4_wave_sum64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// One search phase (asm: it manages the execution mask itself), the pass of
+// K2's drain with the row rules and the tie records added. Registers:
+// v[88:91] / v[92:95] = {f, avail, ld, rd} of the current row, alternating
+// per pass; v[96:97] = the record {item, ld} (v96 holds the lane's item).
+// A pass:
+//   pop    as in K2 (v_cmpx on an empty row, borrow = piece done);
+//   step   b = f & -f; a full board when avail == b. A tie board (the last
+//          queen on endb, or bit b1 of ld: endb = ~0 makes every board of
+//          the item one) leaves the solution mask and is written to the
+//          wave's chunk at index cnt + rank; the others add to sols. Push
+//          as in K2. The child: ld' = (ld + b) << 1, rd' = (rd | b) >> 1,
+//          t1 = ld' >> 31 (arithmetic: the row-b2 marker), t2 = bit n of rd'
+//          (the last-row marker), avail' = (avail ^ b) & ~(lm & t2), and at
+//          row b2 the side rule from the open side columns of avail':
+//          m0 / mt = bit 0 / n-1 of avail' as 0 or ~0, blocked = side
+//          columns where both are open, all others where either is; the
+//          free set is avail' & ~ld' & ~(rd' | (blocked & t1)).
+// bitop3 truth tables index (S0 S1 S2) as a 3-bit number, S0 the high bit:
+// 0x14 = (a ^ b) & ~c, 0xd4 = c ? a & b : a | b, 0xf8 = a | (b & c),
+// 0x10 = a & ~b & ~c.
+#define NQM_K4_POP(U, CX, CT)                              \
+    "v_cmpx_eq_u32 vcc, 0, " CX "\n"                       \
+    "v_subrev_co_u32 %[q], vcc, %[row], %[q]\n"            \
+    "ds_read_b128 " CT ", %[q] offset:%[row]\n"            \
+    "s_cbranch_vccz .Lk4_nd" #U "_%=\n"                    \
+    "s_andn2_b64 %[alive], %[alive], vcc\n"                \
+    "s_bcnt1_i32_b64 %[t], vcc\n"                          \
+    "s_add_u32 %[t2], %[ran], " #U "\n"                    \
+    "s_mul_i32 %[t], %[t], %[t2]\n"                        \
+    "s_add_u32 %[nodes], %[nodes], %[t]\n"                 \
+    ".Lk4_nd" #U "_%=:\n"                                  \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    "s_waitcnt lgkmcnt(0)\n"
+#define NQM_K4_STEP(U, CX, CA, CL, CR, CT, NX, NA, NL, NR) \
+    "v_sub_u32 %[ng], 0, " CX "\n"                         \
+    "v_and_b32 %[b], " CX ", %[ng]\n"                      \
+    "v_or_b32 " NR ", " CR ", %[b]\n"                      \
+    "v_add_lshl_u32 " NL ", %[b], " CL ", 1\n"             \
+    "v_cmp_eq_u32 %[sm], " CA ", %[b]\n"                   \
+    "v_cmpx_ne_u32 vcc, " CX ", %[b]\n"                    \
+    "v_xor_b32 " CX ", " CX ", %[b]\n"                     \
+    "ds_write_b128 %[q], " CT " offset:%[row]\n"           \
+    "v_add_u32 %[q], %[row], %[q]\n"                       \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    "v_and_b32 %[e], %[b], %[endb]\n"                      \
+    "v_and_or_b32 %[e], " CL ", %[bitb1], %[e]\n"          \
+    "v_cmp_ne_u32 %[se], 0, %[e]\n"                        \
+    "v_lshrrev_b32 " NR ", 1, " NR "\n"                    \
+    "s_and_b64 %[se], %[se], %[sm]\n"                      \
+    "s_andn2_b64 %[sm], %[sm], %[se]\n"                    \
+    "s_bcnt1_i32_b64 %[t], %[sm]\n"                        \
+    "s_add_u32 %[sols], %[sols], %[t]\n"                   \
+    "v_ashrrev_i32 %[t1], 31, " NL "\n"                    \
+    "v_bfe_i32 %[t2v], " NR ", %[sn], 1\n"                 \
+    "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
+    "v_bitop3_b32 " NA ", " CA ", %[b], %[t2v] bitop3:0x14\n" \
+    "v_bfe_i32 %[m0], " NA ", 0, 1\n"                      \
+    "v_bfe_i32 %[mt], " NA ", %[snm1], 1\n"                \
+    "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
+    "v_bitop3_b32 %[t1], " NR ", %[mt], %[t1] bitop3:0xf8\n" \
+    "v_bitop3_b32 " NX ", " NA ", " NL ", %[t1] bitop3:0x10\n" \
+    "s_cmp_eq_u64 %[se], 0\n"                              \
+    "s_cbranch_scc1 .Lk4_ne" #U "_%=\n"                    \
+    "s_mov_b64 exec, %[se]\n"                              \
+    "v_mbcnt_lo_u32_b32 %[e], exec_lo, 0\n"                \
+    "v_mbcnt_hi_u32_b32 %[e], exec_hi, %[e]\n"             \
+    "v_add_lshl_u32 %[e], %[e], %[cnt], 3\n"               \
+    "v_mov_b32 v97, " CL "\n"                              \
+    "global_store_dwordx2 %[e], v[96:97], %[recbase]\n"    \
+    "s_bcnt1_i32_b64 %[t], %[se]\n"                        \
+    "s_add_u32 %[cnt], %[cnt], %[t]\n"                     \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    ".Lk4_ne" #U "_%=:\n"
+#define NQM_K4_PASS_A(U) NQM_K4_POP(U, "v88", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95")
+#define NQM_K4_PASS_B(U) NQM_K4_POP(U, "v92", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91")
+#define NQM_K4_PASS8 NQM_K4_PASS_A(0) NQM_K4_PASS_B(1) NQM_K4_PASS_A(2) NQM_K4_PASS_B(3) \
+                     NQM_K4_PASS_A(4) NQM_K4_PASS_B(5) NQM_K4_PASS_A(6) NQM_K4_PASS_B(7)
+
+typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
+typedef uint32_t k4_v2 __attribute__((ext_vector_type(2)));
+
+// Runs passes until at most floor_live lanes hold a piece, lim32 passes ran
+// or the chunk holds kRecLimit records (checked every 8 passes). Returns the
+// non-tie full boards (psols) and the placements (pnodes) of the wave.
+__device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, uint32_t bitb1, uint32_t endb,
+                                         uint32_t item, uint32_t& cnt, uint64_t recbase, uint32_t sn,
+                                         uint32_t snm1, uint32_t sside, uint32_t floor_live, uint32_t lim32,
+                                         uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
+    k4_v4 s4 = {st.x, st.y, st.z, st.w};
+    k4_v4 s5;
+    k4_v2 rec = {item, 0u};
+    uint64_t live, alive, sm, se;
+    uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
+    uint32_t sols = 0, nodes = 0;
+    uint32_t c = cnt;
+    asm volatile(
+        "s_mov_b32 %[sols], 0\n"
+        "s_mov_b32 %[nodes], 0\n"
+        "s_mov_b64 %[live], exec\n"
+        "s_mov_b64 %[alive], exec\n"
+        "s_mov_b32 %[ran], 0\n"
+        ".Lk4_pass_%=:\n" NQM_K4_PASS8
+        "s_add_u32 %[ran], %[ran], 8\n"
+        "s_bcnt1_i32_b64 %[t], exec\n"
+        "s_cmp_le_u32 %[t], %[floor]\n"
+        "s_cbranch_scc1 .Lk4_end_%=\n"
+        "s_cmp_ge_u32 %[cnt], %[clim]\n"
+        "s_cbranch_scc1 .Lk4_end_%=\n"
+        "s_cmp_lt_u32 %[ran], %[lim]\n"
+        "s_cbranch_scc1 .Lk4_pass_%=\n"
+        ".Lk4_end_%=:\n"
+        "s_bcnt1_i32_b64 %[t], exec\n"
+        "s_mul_i32 %[t], %[t], %[ran]\n"
+        "s_add_u32 %[nodes], %[nodes], %[t]\n"
+        "s_mov_b64 exec, %[live]\n"
+        "s_waitcnt lgkmcnt(0)\n"
+        : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:97]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng),
+          [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live),
+          [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
+          [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c)
+        : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [lm] "v"(lm),
+          [bitb1] "v"(bitb1), [endb] "v"(endb), [sn] "s"(sn), [snm1] "s"(snm1), [sside] "s"(sside),
+          [recbase] "s"(recbase)
+        : "vcc", "scc", "memory");
+    (void)s5;
+    (void)rec;
+    st = make_uint4(s4.x, s4.y, s4.z, s4.w);
+    cnt = c;
+    psols = sols;
+    pnodes = nodes;
+}
+
+__global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
+    extern __shared__ uint4 lds[];
+    constexpr uint32_t T = kK4Threads, row = kK4Row;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t base = 16u * tid;
+    uint32_t* const wscratch = reinterpret_cast<uint32_t*>(lds) + wave * 64u;  // in row 0
+    if ((uint32_t)(size_t)(k4_lds_u32*)lds != 0u) {  // the slots are addressed from 0
+        if (tid == 0) atomicOr(a.err, 16u);
+        return;
+    }
+    const uint32_t count = a.count;
+    const uint32_t waves = gridDim.x * (T / 64u);
+
+    uint4 st = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t q = base;
+    uint32_t lm = 0, bitb1 = 0, endb = 0, item = 0;
+    auto holds = [&]() { return st.x != 0u || q != base; };
+
+    // the queue: nseg segments with a head each (as in K2's drain)
+    const uint32_t nseg = a.nseg;
+    uint32_t seg = blockIdx.x % nseg, seg_done = 0;
+    auto seg_lo = [&](uint32_t sg) { return (uint32_t)((uint64_t)count * sg / nseg); };
+    const uint32_t seg_waves = max(1u, waves / nseg);
+    bool queue_open = true;
+    uint32_t lq = 0, lq_end = 0, rem = seg_lo(seg + 1) - seg_lo(seg);
+
+    // the wave's record chunk (taken before its first phase)
+    constexpr uint32_t kNoChunk = 0xffffffffu;
+    uint32_t chunk = kNoChunk, cnt = 0;
+    auto grab_chunk = [&]() {
+        uint32_t ch = 0;
+        if (lane == 0) ch = atomicAdd(a.chunk_head, 1u);
+        ch = __builtin_amdgcn_readfirstlane(__shfl(ch, 0, 64));
+        if (ch >= a.nchunks) {  // out of chunks: the sink chunk (the host retries with more)
+            if (lane == 0) atomicOr(a.err, 64u);
+            ch = a.nchunks;
+        }
+        chunk = ch;
+        cnt = 0;
+    };
+    auto close_chunk = [&]() {
+        if (lane == 0) {
+            a.rec_counts[chunk] = cnt;
+            if (cnt) atomicAdd(a.rec_total, cnt);
+        }
+    };
+
+    auto take = [&](uint32_t idx) {
+        const SymQ it = a.items[idx];
+        const uint32_t r = symq_row(it.meta), b1 = symq_b1(it.meta);
+        uint32_t ld = it.ld, rd = it.rd;
+        lm = 0;
+        bitb1 = 0;
+        endb = 0;
+        if (!symq_corner(it.meta)) {
+            const SymCase c = sym_case(a.n, b1, 0u);
+            if (c.b2 > r) ld |= 1u << (31u - (c.b2 - r));  // reaches bit 31 at child row b2
+            rd |= 1u << (a.n + (a.last - r));              // reaches bit n at the last row
+            lm = c.lastmask;
+            bitb1 = 1u << b1;
+            endb = (b1 < r && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
+        }
+        st = make_uint4(it.f, ~it.cols & a.mask, ld, rd);
+        q = base;
+        item = idx;
+    };
+
+    uint64_t leaves = 0, exec_nodes = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!holds());
+        if (idle) {
+            const uint32_t nidle = __popcll(idle);
+            if (queue_open || lq < lq_end) {
+                const uint32_t take_local = min(nidle, lq_end - lq);
+                uint32_t g = 0, got = 0;
+                if (take_local < nidle && queue_open) {
+                    const uint32_t need = nidle - take_local;
+                    uint32_t want = rem / (2u * seg_waves);
+                    want = want < need ? need : (want > kK4Grab ? kK4Grab : want);
+                    const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
+                    if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, want);
+                    g = __shfl(g, 0, 64);
+                    got = g >= len ? 0u : min(want, len - g);
+                    rem = g >= len ? 0u : len - g - got;
+                    g += lo;
+                    if (got < want) {
+                        if (++seg_done == nseg) queue_open = false;
+                        seg = seg + 1u == nseg ? 0u : seg + 1u;
+                        rem = seg_lo(seg + 1) - seg_lo(seg);
+                    }
+                }
+                const uint32_t r = k4_rank(idle);
+                uint32_t idx = 0xffffffffu;
+                if (!holds()) {
+                    if (r < take_local)
+                        idx = lq + r;
+                    else if (r - take_local < got)
+                        idx = g + (r - take_local);
+                }
+                if (idx != 0xffffffffu) take(idx);
+                lq += take_local;
+                if (got) {
+                    lq = g + min(got, nidle - take_local);
+                    lq_end = g + got;
+                }
+                idle = __ballot(!holds());
+            }
+            if (idle) {
+                // donation (as in K2): a lane with an open row gives its
+                // bottom one, with its item and rule constants
+                const bool is_donor = q != base;
+                const uint64_t donors = __ballot(is_donor);
+                const uint32_t k = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
+                if (k) {
+                    const uint32_t dr = k4_rank(donors);
+                    const bool gives = is_donor && dr < k;
+                    if (gives) wscratch[dr] = lane;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const uint32_t ir = k4_rank(idle);
+                    const bool recv = !holds() && ir < k;
+                    const uint32_t dl = recv ? wscratch[ir] : lane;
+                    const uint32_t d_lm = __shfl(lm, dl, 64), d_bitb1 = __shfl(bitb1, dl, 64);
+                    const uint32_t d_endb = __shfl(endb, dl, 64), d_item = __shfl(item, dl, 64);
+                    if (recv) {
+                        st = k4_slot_load(base - 16u * lane + 16u * dl + row);  // the donor's slot 0
+                        lm = d_lm;
+                        bitb1 = d_bitb1;
+                        endb = d_endb;
+                        item = d_item;
+                        q = base;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (gives) {
+                        for (uint32_t s = base + 2u * row; s <= q; s += row) k4_slot_store(s - row, k4_slot_load(s));
+                        q -= row;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+        }
+        const uint64_t live_mask = __ballot(holds());
+        const bool have_items = queue_open || lq < lq_end;
+        if (live_mask == 0 && !have_items) break;
+
+        const uint32_t nlive = __popcll(live_mask);
+        const bool can_refill = have_items || __ballot(q != base) != 0;
+        uint32_t floor_live = 0, limit = 1u << 25;  // 64 lanes x 2^25 passes fit the u32 counts
+        if (can_refill) {
+            if (nlive > 64u - a.refill_idle)
+                floor_live = 64u - a.refill_idle;
+            else
+                limit = 256u;
+        }
+        floor_live = __builtin_amdgcn_readfirstlane(floor_live);
+        limit = __builtin_amdgcn_readfirstlane(limit);
+        if (chunk == kNoChunk) grab_chunk();
+        const uint64_t rb = (uint64_t)(size_t)(a.recs + (size_t)chunk * kRecChunk);
+        // (readfirstlane returns int: widen through uint32_t, never by sign)
+        const uint64_t recbase = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(rb >> 32)) << 32) |
+                                 (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)rb);
+        uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
+        if (holds()) {
+            k4_phase(st, q, lm, bitb1, endb, item, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit, psols,
+                     pnodes, ran);
+            if ((int32_t)q < 0) {  // the piece ended: its last load is void
+                q = base;
+                st.x = 0;
+            }
+        }
+        c = k4_wave_max(c);
+        // the wave's counts come from scalar registers: one lane keeps them
+        const uint32_t lead = (uint32_t)__ffsll((long long)live_mask) - 1u;
+        psols = __shfl(psols, lead, 64);
+        pnodes = __shfl(pnodes, lead, 64);
+        leaves += psols;
+        exec_nodes += pnodes;
+        cnt = c;
+        if (cnt >= kRecLimit) {
+            close_chunk();
+            grab_chunk();
+        }
+    }
+    if (chunk != kNoChunk) close_chunk();
+    if (lane == 0) {
+        uint64_t* const r = a.wave_part + 2u * (blockIdx.x * (T / 64u) + wave);
+        r[0] = leaves;
+        r[1] = exec_nodes;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ties: search the recorded boards again and run the rotation check
+// ---------------------------------------------------------------------------
+constexpr uint64_t kEmptyKey = ~0ull;
+
+__device__ __forceinline__ bool k4_claim(unsigned long long* table, uint32_t tmask, uint64_t key) {
+    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & tmask;
+    for (;;) {
+        const unsigned long long old = atomicCAS(&table[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
+        if (old == kEmptyKey) return true;
+        if (old == key) return false;
+        h = (h + 1u) & tmask;
+    }
+}
+
+// The board of a search again: rows below r0 from the item's prefix, the
+// others from the path (5 bits a row from r0 on).
+struct PathBoard {
+    uint64_t pre, path;
+    uint32_t r0;
+    __device__ uint32_t operator()(int r) const {
+        const uint32_t u = (uint32_t)r;
+        const uint64_t v = u < r0 ? pre >> (5u * u) : path >> (5u * (u - r0));
+        return 1u << (uint32_t)(v & 31u);
+    }
+};
+
+// Every board of `it`'s subtree whose last-row ld equals L and which the
+// drain would have recorded: (classes, total weight) after the rotation
+// check. Rows above the last keep only the columns whose diagonal bit is in
+// L. Going back up a row rebuilds that row's cols (cols ^ b) and ld (ld >> 1
+// without b: the bit 31 this loses, and those lost going up further, end
+// above bit 31 by the last row); rd cannot be rebuilt (its bit 0, lost on
+// the way down, reaches the children after a second step up), so it is kept
+// per row with the candidates left.
+__device__ void k4_research(const SymQ& it, uint32_t L, uint64_t& uq, uint64_t& tot) {
+    const uint32_t n = symq_n(it.meta), r0 = symq_row(it.meta), b1 = symq_b1(it.meta);
+    const SymCase c = sym_case(n, b1, 0u);
+    const uint32_t endb = (b1 < r0 && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
+    const uint32_t bitb1 = 1u << b1;
+    PathBoard brd{((uint64_t)it.p1 << 32) | it.p0, 0u, r0};
+    uint32_t sf[kK4MaxRows], sr[kK4MaxRows];  // rows r0..last-1
+    uint32_t y = r0, l = it.ld, d = it.cols, r = it.rd;
+    auto keep = [&](uint32_t yy) { return yy < c.last ? (L >> (c.last - yy)) & c.mask : c.mask; };
+    uint32_t f = it.f & keep(y);
+    for (;;) {
+        if (f == 0u) {
+            if (y == r0) break;
+            --y;
+            const uint32_t b = 1u << (uint32_t)((brd.path >> (5u * (y - r0))) & 31u);
+            d ^= b;
+            l = (l >> 1) & ~b;
+            r = sr[y - r0];
+            f = sf[y - r0];
+            continue;
+        }
+        const uint32_t bit = f & (0u - f);
+        f ^= bit;
+        const uint32_t sh = 5u * (y - r0);
+        brd.path = (brd.path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
+        if (y == c.last) {
+            if (l == L && ((bit & endb) | (l & bitb1)) != 0u) {
+                const uint32_t w = sym_check(c, brd);
+                uq += w != 0u;
+                tot += w;
+            }
+            continue;
+        }
+        sf[y - r0] = f;
+        sr[y - r0] = r;
+        l = (l | bit) << 1;
+        d |= bit;
+        r = (r | bit) >> 1;
+        ++y;
+        f = sym_free(c, y, l, d, r) & keep(y);
+    }
+}
+
+// Chunk ch holds rec_counts[ch] records; the first copy of each {item, ld}
+// (hash set) searches again.
+__global__ void __launch_bounds__(256)
+k4_ties_kernel(const uint2* __restrict__ recs, const uint32_t* __restrict__ rec_counts, uint32_t nchunks,
+               const SymQ* __restrict__ items, unsigned long long* __restrict__ table, uint32_t tmask,
+               unsigned long long* __restrict__ out) {
+    uint64_t uq = 0, tot = 0;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const uint32_t m = rec_counts[ch];
+        const uint2* const rp = recs + (size_t)ch * kRecChunk;
+        for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
+            const uint2 rc = rp[j];
+            const uint64_t key = ((uint64_t)rc.x << 32) | rc.y;
+            if (!k4_claim(table, tmask, key)) continue;
+            k4_research(items[rc.x], rc.y, uq, tot);
+        }
+    }
+    uq = k4_wave_sum64(uq);
+    tot = k4_wave_sum64(tot);
+    if ((threadIdx.x & 63u) == 0u && (uq | tot)) {
+        atomicAdd(&out[0], (unsigned long long)uq);
+        atomicAdd(&out[1], (unsigned long long)tot);
+    }
+}
+
+__global__ void k4_fill_u64_kernel(unsigned long long* p, size_t count, unsigned long long v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+struct K4Host {
+    std::vector<SymQ> items;
+    uint64_t nodes = 0;
+};
+
+void k4_host_rec(K4Host& h, const SymCase& c, uint32_t y, uint32_t l, uint32_t d, uint32_t r, uint64_t p,
+                 uint32_t top_b1, uint32_t depth) {
+    const uint32_t f = sym_free(c, y, l, d, r);
+    if (!f) return;
+    if (y == depth) {
+        SymQ it;
+        it.f = f;
+        it.cols = d;
+        it.ld = l & c.mask;
+        it.rd = r;
+        it.meta = symq_meta(c.n, y, c.b1, c.corner, top_b1);
+        it.pad = 0;
+        it.p0 = (uint32_t)p;
+        it.p1 = (uint32_t)(p >> 32);
+        h.items.push_back(it);
+        return;
+    }
+    for (uint32_t g = f; g; g &= g - 1u) {
+        const uint32_t bit = g & (0u - g);
+        ++h.nodes;
+        const uint64_t p2 = y < kPrefixRows ? p | ((uint64_t)__builtin_ctz(bit) << (5u * y)) : p;
+        const uint32_t tb = top_b1 | (!c.corner && y == c.b1 && bit == c.top ? 1u : 0u);
+        k4_host_rec(h, c, y + 1u, ((l | bit) << 1) & c.mask, d | bit, (r | bit) >> 1, p2, tb, depth);
+    }
+}
+
+}  // namespace
+
+// The expansion row: rows < R hold every "rows < b1" side rule of a
+// non-corner case (b1 <= (n-3)/2 < R).
+uint32_t k4_rows_done(uint32_t n) { return (n & 1u) ? (n - 1u) / 2u : n / 2u - 1u; }
+
+bool k4_supports(uint32_t n) {
+    if (n < 10 || n > 21) return false;
+    const uint32_t R = k4_rows_done(n), last = n - 1u;
+    // the rd marker n + (last - R) fits in 32 bits; real ld bits (at most
+    // n - 1 + (last - R)) stay below bit 31, and below the row-b2 marker,
+    // which starts 31 - (b2 - R) bits up with b2 <= n - 2; the stack of
+    // last - R rows fits LDS at 16 waves per CU
+    return n + (last - R) <= 31u && n - 1u + (last - R) < 31u && n - 2u < 32u - n + R && last - R <= 9u;
+}
+
+// Grows b to `bytes`, keeping its first `used` bytes (stream-ordered copy).
+static int grow_keep(nqm_ctx* ctx, DevBuf& b, size_t used, size_t bytes) {
+    if (b.cap >= bytes) return NQM_OK;
+    DevBuf nb;
+    int rc = ensure_dev(ctx, nb, bytes + bytes / 2);
+    if (rc != NQM_OK) return rc;
+    if (used) NQM_HIP(ctx, hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, ctx->stream));
+    if (b.ptr) {
+        NQM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        NQM_HIP(ctx, hipFree(b.ptr));
+    }
+    b = nb;
+    return NQM_OK;
+}
+
+int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
+    const double t0 = now_ms();
+    if (!k4_supports(n)) return fail(ctx, NQM_EUNSUPPORTED, "the symmetric drain takes 10 <= n <= 21, not %u", n);
+    NQM_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    SymWork& w = ctx->sym;
+    const uint32_t R = k4_rows_done(n), last = n - 1u;
+    const uint32_t depth0 = std::min(R, 4u);
+
+    // host rows: corner (rows 0 and 1 fixed), non-corner (row 0 fixed)
+    K4Host h;
+    try {
+        for (uint32_t b1 = 2; b1 < last; ++b1) {
+            const SymCase c = sym_case(n, b1, 1u);
+            const uint32_t bit = 1u << b1;
+            h.nodes += 2;
+            k4_host_rec(h, c, 2, ((2u | bit) << 1) & c.mask, 1u | bit, bit >> 1, (uint64_t)b1 << 5, 0u, depth0);
+        }
+        for (uint32_t b1 = 1; b1 < last - b1; ++b1) {
+            const SymCase c = sym_case(n, b1, 0u);
+            const uint32_t bit = 1u << b1;
+            h.nodes += 1;
+            k4_host_rec(h, c, 1, (bit << 1) & c.mask, bit, bit >> 1, (uint64_t)b1, 0u, depth0);
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, NQM_ENOMEM, "host items of the symmetric drain");
+    }
+    // items already at their target row start the final queue, the others
+    // are the first pending level
+    std::vector<SymQ> hq;
+    try {
+        hq.reserve(h.items.size());
+        for (const SymQ& it : h.items)
+            if (symq_row(it.meta) >= symq_target(it.meta, R)) hq.push_back(it);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, NQM_ENOMEM, "host items of the symmetric drain");
+    }
+    const uint32_t host_fin = (uint32_t)hq.size();
+    for (const SymQ& it : h.items)
+        if (symq_row(it.meta) < symq_target(it.meta, R)) hq.push_back(it);
+    const uint32_t host_pend = (uint32_t)hq.size() - host_fin;
+
+    int rc;
+    // counters (u32 index): [0..1] level count, [2] final cursor, [3] next
+    // cursor, [8..9] u64 expansion nodes, [16] err, [32..35] u64 tie sums,
+    // [64] chunk head, [65] records written, [128..] queue heads
+    if ((rc = ensure_dev(ctx, w.ctr, 4096)) != NQM_OK) return rc;
+    uint32_t* const d_cnt = static_cast<uint32_t*>(w.ctr.ptr);
+    unsigned long long* const d_xnodes = reinterpret_cast<unsigned long long*>(d_cnt + 8);
+    uint32_t* const d_err = d_cnt + 16;
+    unsigned long long* const d_ties = reinterpret_cast<unsigned long long*>(d_cnt + 32);
+    uint32_t* const d_chunk_head = d_cnt + 64;
+    uint32_t* const d_rec_total = d_cnt + 65;
+    uint32_t* const d_head = d_cnt + 128;
+    static_assert(128 + kHeadSegs * kHeadStride <= 1024, "heads fit the counter page");
+    if ((rc = ensure_dev(ctx, w.items, std::max<size_t>(hq.size(), 1) * sizeof(SymQ))) != NQM_OK) return rc;
+
+    const bool known = w.sizes_n == n;
+    uint64_t nfin64 = host_fin;
+    if (known) {
+        for (uint32_t v : w.level_fin) nfin64 += v;
+        if ((rc = ensure_dev(ctx, w.fin, (size_t)std::max<uint64_t>(nfin64, 1) * sizeof(SymQ))) != NQM_OK) return rc;
+        for (size_t L = 0; L < w.level_in.size(); ++L)
+            if ((rc = ensure_dev(ctx, w.pend[(L + 1) & 1], (size_t)std::max<uint32_t>(
+                                     L + 1 < w.level_in.size() ? w.level_in[L + 1] : 1u, 1u) * sizeof(SymQ))) != NQM_OK)
+                return rc;
+    }
+
+    NQM_HIP(ctx, hipEventRecord(ctx->ev_x0, st));
+    NQM_HIP(ctx, hipMemsetAsync(d_cnt, 0, 4096, st));
+    NQM_HIP(ctx, hipMemcpyAsync(w.items.ptr, hq.data(), hq.size() * sizeof(SymQ), hipMemcpyHostToDevice, st));
+    // the final queue starts with the host's final items
+    if (!known && (rc = grow_keep(ctx, w.fin, 0, (size_t)std::max<uint32_t>(host_fin, 1) * sizeof(SymQ))) != NQM_OK)
+        return rc;
+    if (host_fin) {
+        NQM_HIP(ctx, hipMemcpyAsync(w.fin.ptr, w.items.ptr, (size_t)host_fin * sizeof(SymQ),
+                                    hipMemcpyDeviceToDevice, st));
+        NQM_HIP(ctx, hipMemcpyAsync(d_cnt + 2, &host_fin, 4, hipMemcpyHostToDevice, st));
+    }
+    // levels: every pending item grows by one row; children at their target
+    // row are appended to the final queue (cursor d_cnt[2]), the others form
+    // the next level (cursor d_cnt[3])
+    std::vector<uint32_t> level_in, level_fin;
+    const SymQ* pend = static_cast<const SymQ*>(w.items.ptr) + host_fin;
+    uint32_t npend = host_pend;
+    uint32_t L = 0;
+    while (npend) {
+        const uint32_t grid = std::min<uint32_t>(4096u, (npend + 255u) / 256u);
+        uint32_t nfin_l = 0, nnext = 0;
+        if (known) {
+            nfin_l = w.level_fin[L];
+            nnext = L + 1 < w.level_in.size() ? w.level_in[L + 1] : 0u;
+        } else {
+            NQM_HIP(ctx, hipMemsetAsync(d_cnt, 0, 8, st));
+            hipLaunchKernelGGL(k4_expand_count_kernel, dim3(grid), dim3(256), 0, st, pend, npend, R, d_cnt);
+            NQM_HIP(ctx, hipGetLastError());
+            uint32_t hc[2] = {0, 0};
+            NQM_HIP(ctx, hipMemcpyAsync(hc, d_cnt, 8, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            nfin_l = hc[0];
+            nnext = hc[1];
+            if (nfin64 + nfin_l > 0x7fffffffu || nnext > 0x7fffffffu)
+                return fail(ctx, NQM_ERANGE, "symmetric expansion exceeds 2^31 items");
+            if ((rc = grow_keep(ctx, w.fin, nfin64 * sizeof(SymQ), (size_t)(nfin64 + nfin_l) * sizeof(SymQ))) != NQM_OK)
+                return rc;
+            if ((rc = ensure_dev(ctx, w.pend[(L + 1) & 1], (size_t)std::max(nnext, 1u) * sizeof(SymQ))) != NQM_OK)
+                return rc;
+            level_in.push_back(npend);
+            level_fin.push_back(nfin_l);
+            nfin64 += nfin_l;
+        }
+        SymQ* const nxt = static_cast<SymQ*>(w.pend[(L + 1) & 1].ptr);
+        NQM_HIP(ctx, hipMemsetAsync(d_cnt + 3, 0, 4, st));
+        hipLaunchKernelGGL(k4_expand_write_kernel, dim3(grid), dim3(256), 0, st, pend, npend, R,
+                           static_cast<SymQ*>(w.fin.ptr), (uint32_t)nfin64,
+                           nxt, nnext, d_cnt, d_xnodes, d_err);
+        NQM_HIP(ctx, hipGetLastError());
+        pend = nxt;
+        npend = nnext;
+        ++L;
+    }
+    if (!known) {
+        w.sizes_n = 0;
+        w.host_fin = host_fin;
+        w.level_in = level_in;
+        w.level_fin = level_fin;
+        w.chunks = 0;
+    }
+    const uint32_t nfin = (uint32_t)nfin64;
+    const SymQ* const items = static_cast<const SymQ*>(w.fin.ptr);
+
+    // drain shape: 16 waves per CU when the stack fits (4 workgroups of 4 waves)
+    const uint32_t stack_rows = last - R;  // rows R..last-1 push; deeper corner items need fewer
+    const size_t lds = (size_t)(stack_rows + 1u) * kK4Row;
+    if (lds > ctx->lds_per_block) return fail(ctx, NQM_EUNSUPPORTED, "symmetric stack does not fit LDS");
+    const uint32_t per_cu = std::max<uint32_t>(1u, std::min<uint32_t>(4u, (uint32_t)(ctx->lds_per_cu / lds)));
+    const uint32_t blocks = (uint32_t)ctx->prop.multiProcessorCount * per_cu;
+    const uint32_t nwaves = blocks * (kK4Threads / 64u);
+    if ((rc = ensure_dev(ctx, w.wave, (size_t)nwaves * 16u)) != NQM_OK) return rc;
+
+    // record chunks: as many as the last count of this n used, else an
+    // estimate from the queue (a run that runs out sets err bit 64 and is
+    // repeated with 4x more)
+    uint64_t nchunks = w.chunks ? w.chunks + w.chunks / 4u + 64u : (uint64_t)nwaves + nfin64 / 4096u + 64u;
+    uint64_t leaves = 0, dnodes = 0, tie_u = 0, tie_t = 0, xnodes = 0;
+    float search_ms = 0.f, all_ms = 0.f;
+    for (int attempt = 0;; ++attempt) {
+        if (nchunks * kRecChunk > 0xffffffffull) return fail(ctx, NQM_ERANGE, "too many tie records");
+        if ((rc = ensure_dev(ctx, w.recs, (size_t)(nchunks + 1u) * kRecChunk * sizeof(uint2))) != NQM_OK) return rc;
+        if ((rc = ensure_dev(ctx, w.rec_counts, (size_t)(nchunks + 1u) * 4u)) != NQM_OK) return rc;
+        NQM_HIP(ctx, hipMemsetAsync(d_chunk_head, 0, 8, st));  // chunk head, records written
+        NQM_HIP(ctx, hipMemsetAsync(d_head, 0, kHeadSegs * kHeadStride * 4, st));
+        NQM_HIP(ctx, hipMemsetAsync(d_ties, 0, 16, st));
+        K4Args a;
+        a.items = items;
+        a.count = nfin;
+        a.nseg = kHeadSegs;
+        a.head = d_head;
+        a.n = n;
+        a.last = last;
+        a.mask = (1u << n) - 1u;
+        a.side = (1u << last) | 1u;
+        a.refill_idle = 8u;
+        a.recs = static_cast<uint2*>(w.recs.ptr);
+        a.rec_counts = static_cast<uint32_t*>(w.rec_counts.ptr);
+        a.chunk_head = d_chunk_head;
+        a.rec_total = d_rec_total;
+        a.nchunks = (uint32_t)nchunks;
+        a.wave_part = static_cast<uint64_t*>(w.wave.ptr);
+        a.err = d_err;
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_start, st));
+        if (nfin) {
+            hipLaunchKernelGGL(k4_drain_kernel, dim3(blocks), dim3(kK4Threads), lds, st, a);
+            NQM_HIP(ctx, hipGetLastError());
+        }
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
+        uint32_t hc[2] = {0, 0}, err = 0;  // chunks used, records written
+        NQM_HIP(ctx, hipMemcpyAsync(hc, d_chunk_head, 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        if (err & 1u) return fail(ctx, NQM_EDEVICE, "symmetric expansion overflowed its counted size");
+        if (err & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
+        if (err & 64u) {
+            if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
+            nchunks *= 4u;
+            NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
+            continue;
+        }
+        NQM_HIP(ctx, hipEventElapsedTime(&search_ms, ctx->ev_start, ctx->ev_stop));
+        const uint32_t used = hc[0];
+        // ties: a hash set of {item, ld}, at least twice the records
+        uint32_t tbits = 10;
+        while ((1ull << tbits) < 2ull * hc[1] && tbits < 31) ++tbits;
+        if ((rc = ensure_dev(ctx, w.table, (size_t)8u << tbits)) != NQM_OK) return rc;
+        unsigned long long* const table = static_cast<unsigned long long*>(w.table.ptr);
+        if (hc[1]) {
+            hipLaunchKernelGGL(k4_fill_u64_kernel, dim3(2048), dim3(256), 0, st, table, (size_t)1 << tbits,
+                               (unsigned long long)kEmptyKey);
+            NQM_HIP(ctx, hipGetLastError());
+            const uint32_t grid = std::min<uint32_t>(used, 16384u);
+            hipLaunchKernelGGL(k4_ties_kernel, dim3(grid), dim3(256), 0, st, static_cast<const uint2*>(w.recs.ptr),
+                               static_cast<const uint32_t*>(w.rec_counts.ptr), used, items, table,
+                               (uint32_t)((1ull << tbits) - 1u), d_ties);
+            NQM_HIP(ctx, hipGetLastError());
+        }
+        NQM_HIP(ctx, hipEventRecord(ctx->ev_x1, st));
+        std::vector<uint64_t> wp((size_t)nwaves * 2u);
+        uint64_t hs[2] = {0, 0};
+        unsigned long long hx = 0;
+        NQM_HIP(ctx, hipMemcpyAsync(wp.data(), w.wave.ptr, wp.size() * 8u, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(hs, d_ties, 16, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&hx, d_xnodes, 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipStreamSynchronize(st));
+        NQM_HIP(ctx, hipEventElapsedTime(&all_ms, ctx->ev_x0, ctx->ev_x1));
+        for (uint32_t wv = 0; wv < nwaves; ++wv) {
+            leaves += wp[2u * wv];
+            dnodes += wp[2u * wv + 1u];
+        }
+        tie_u = hs[0];
+        tie_t = hs[1];
+        xnodes = hx;
+        w.chunks = used;
+        break;
+    }
+    w.sizes_n = n;  // the sizes (and the chunk count) of this n are now known
+    const uint64_t u = leaves + tie_u, t = 8u * leaves + tie_t;
+    if (unique) *unique = u;
+    if (total) *total = t;
+    if (stats) {
+        stats->total = t;
+        stats->nodes = h.nodes + xnodes + dnodes;
+        stats->kernel_ms = all_ms;
+        stats->search_ms = search_ms;
+        stats->wall_ms = now_ms() - t0;
+        stats->launches = (known ? 1u : 2u) * L + 3u;
+        stats->blocks = blocks;
+        stats->rounds = 1;
+        stats->lanes = (uint64_t)blocks * kK4Threads;
+    }
+    return NQM_OK;
+}
+
+}  // namespace nqm

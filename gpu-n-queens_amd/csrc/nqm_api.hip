@@ -218,6 +218,10 @@ void nqm_destroy(nqm_ctx* ctx) {
                            &w.part, &w.wave_part, &w.cont[0], &w.cont[1], &w.cont_counts, &w.feats, &w.pred,
                            &w.hist, &w.stat, &w.goal})
         free_dev(*d);
+    nqm::SymWork& sw = ctx->sym;
+    for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.recs, &sw.rec_counts,
+                           &sw.table, &sw.wave})
+        free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

@@ -52,6 +52,18 @@ struct BatchWork {
     GraphPlan plan;
 };
 void drop_plan(GraphPlan& plan);
+
+// Device buffers of the symmetric drain (k4_sym.hip), kept across calls
+// (they only grow), and the expansion sizes of the last n it counted (the
+// tree depends on n only, so a repeated n launches the levels without
+// counting them first or waiting for the host).
+struct SymWork {
+    DevBuf ctr, items, fin, pend[2], recs, rec_counts, table, wave;
+    uint32_t sizes_n = 0;                         // 0: no sizes kept
+    uint32_t host_fin = 0;                        // items the host puts in the final queue
+    std::vector<uint32_t> level_in, level_fin;    // per level: pending items in, final items out
+    uint64_t chunks = 0;                          // record chunks the last count of sizes_n used
+};
 }  // namespace nqm
 
 struct nqm_ctx {
@@ -64,6 +76,7 @@ struct nqm_ctx {
     uint32_t lds_per_cu = 0;
     nqm::DevBuf pool, part, err;
     nqm::BatchWork batch;
+    nqm::SymWork sym;
     nqm::HostBuf h_part;
     std::string last_error;
 };
@@ -86,6 +99,10 @@ int ensure_dev(nqm_ctx* ctx, DevBuf& b, size_t bytes);
 int ensure_host(nqm_ctx* ctx, HostBuf& b, size_t bytes);
 double now_ms();
 
+
+// ---- K4: symmetry-reduced count on a persistent drain (k4_sym.hip) --------
+bool k4_supports(uint32_t n);
+int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats);
 
 // Looks up the device address of the allocation-free count's scratch array
 // for `device` (current), once per device (nqm_static.hip).

@@ -1,106 +1,25 @@
-// sym.hip -- symmetry-reduced counting ("transformation elimination").
+// sym.hip -- symmetry-reduced counting ("transformation elimination"), one
+// thread per item (K3).
 //
 // The reference lists it as future work (README.md:38-45) and counts every
 // solution of the mirror-halved tree. This path counts each class of
-// solutions under the 8 symmetries of the board once, by the published
-// corner / non-corner method, and weights it by its orbit size (2, 4 or 8):
-//   - corner case: a queen at (0,0); rows 2..b1-1 avoid column 1, where b1 is
-//     row 1's column; no such solution is symmetric, so each counts 8;
-//   - non-corner case: row 0's queen at column b1 < b2 = n-1-b1; rows above
-//     b1 avoid the side columns, row b2 completes them, the last row avoids
-//     the columns within b1 of a corner, and a full board counts only if no
-//     rotation of it is smaller (2, 4 or 8 as it equals its 90 or 180 degree
-//     rotation or neither).
-// The host enumerates the first rows (same rules) into items; each lane of
-// k3_sym_kernel searches one item with an LDS stack and keeps its board in
-// LDS for the rotation check. Totals equal the reference's counts; the node
-// numbers are the reduced search's own.
+// solutions under the 8 symmetries of the board once, by the corner /
+// non-corner method of sym_rules.h, and weights it by its orbit size (2, 4
+// or 8). The host enumerates the first rows (same rules) into items; each
+// lane of k3_sym_kernel searches one item with an LDS stack and keeps its
+// board in LDS for the rotation check. Totals equal the reference's counts;
+// the node numbers are the reduced search's own. The persistent drain of
+// k4_sym.hip counts the same tree faster; this kernel stays as its
+// reference on the device and serves the boards the drain does not take.
 #include <algorithm>
 #include <new>
 #include <vector>
 
 #include "nqm_internal.h"
+#include "sym_rules.h"
 
 namespace nqm {
 namespace {
-
-struct SymCase {
-    uint32_t n, last, mask, top, side, lastmask, endbit, b1, b2, corner;
-};
-
-__host__ __device__ inline SymCase sym_case(uint32_t n, uint32_t b1, uint32_t corner) {
-    SymCase c;
-    c.n = n;
-    c.last = n - 1u;
-    c.mask = (1u << n) - 1u;
-    c.top = 1u << (n - 1u);
-    c.side = c.top | 1u;
-    c.b1 = b1;
-    c.b2 = c.last - b1;
-    c.corner = corner;
-    const uint32_t low = (1u << b1) - 1u;  // columns within b1 of a corner
-    c.lastmask = low | (low << (n - b1));
-    c.endbit = c.top >> b1;
-    return c;
-}
-
-// Candidates of row y under the case's rules (0: the row ends the branch).
-// The rules are selects, not branches (rows y < b1, y == b2 and y == last
-// are distinct: b1 < b2 < last), so lanes at different rows stay together.
-__host__ __device__ inline uint32_t sym_free(const SymCase& c, uint32_t y, uint32_t l, uint32_t d, uint32_t r) {
-    uint32_t f = c.mask & ~(l | d | r);
-    // rows above b1: the corner case avoids column 1, the other the side columns
-    f &= (y < c.b1) ? ~(c.corner ? 2u : c.side) : ~0u;
-    if (!c.corner) {
-        // row b2: a side column left to the rows below ends the branch, one
-        // side column taken makes this row take the other
-        const uint32_t ds = d & c.side;
-        const uint32_t at_b2 = ds == 0u ? 0u : (ds != c.side ? (f & c.side) : f);
-        f = y == c.b2 ? at_b2 : f;
-        // the last row avoids the columns within b1 of a corner
-        f = (y == c.last && (f & c.lastmask) != 0u) ? 0u : f;
-    }
-    return f;
-}
-
-// 2, 4 or 8 if the full board b (one-hot column per row) is the smallest of
-// its rotations, as it equals its 90 or 180 degree rotation or neither; 0 if
-// a rotation is smaller.
-template <class Board>
-__host__ __device__ inline uint32_t sym_check(const SymCase& c, const Board& b) {
-    const int last = (int)c.last;
-    if (b(c.b2) == 1u) {  // 90 degrees
-        int own;
-        uint32_t ptn = 2u;
-        for (own = 1; own <= last; ++own, ptn <<= 1) {
-            uint32_t bit = 1u;
-            for (int you = last; b(you) != ptn && b(own) >= bit; --you) bit <<= 1;
-            if (b(own) > bit) return 0u;
-            if (b(own) < bit) break;
-        }
-        if (own > last) return 2u;
-    }
-    if (b(last) == c.endbit) {  // 180 degrees
-        int own, you;
-        for (own = 1, you = last - 1; own <= last; ++own, --you) {
-            uint32_t bit = 1u;
-            for (uint32_t ptn = c.top; ptn != b(you) && b(own) >= bit; ptn >>= 1) bit <<= 1;
-            if (b(own) > bit) return 0u;
-            if (b(own) < bit) break;
-        }
-        if (own > last) return 4u;
-    }
-    if (b(c.b1) == c.top) {  // 270 degrees
-        uint32_t ptn = c.top >> 1;
-        for (int own = 1; own <= last; ++own, ptn >>= 1) {
-            uint32_t bit = 1u;
-            for (int you = 0; b(you) != ptn && b(own) >= bit; ++you) bit <<= 1;
-            if (b(own) > bit) return 0u;
-            if (b(own) < bit) break;
-        }
-    }
-    return 8u;
-}
 
 // One item: the case, the next row y and its masks, the columns of rows < y.
 struct SymItem {
@@ -271,9 +190,10 @@ void sym_host_rec(SymHost& h, const SymCase& c, uint32_t y, uint32_t l, uint32_t
 
 using namespace nqm;
 
-extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
-    if (!ctx) return NQM_EINVAL;
-    if (n < 1 || n > kMaxN) return fail(ctx, NQM_EINVAL, "n=%u outside 1..%u", n, kMaxN);
+namespace {
+
+// K3: host items, one thread each.
+int k3_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
     const double t0 = now_ms();
     uint64_t host_unique = 0, host_total = 0;
     SymHost h;
@@ -365,4 +285,19 @@ extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, u
         stats->blocks = blocks;
     }
     return NQM_OK;
+}
+
+}  // namespace
+
+extern "C" int nqm_count_symmetric_path(nqm_ctx* ctx, uint32_t n, uint32_t path, uint64_t* unique, uint64_t* total,
+                                        nqm_stats* stats) {
+    if (!ctx) return NQM_EINVAL;
+    if (n < 1 || n > kMaxN) return fail(ctx, NQM_EINVAL, "n=%u outside 1..%u", n, kMaxN);
+    if (path > NQM_SYM_DRAIN) return fail(ctx, NQM_EINVAL, "unknown symmetric path %u", path);
+    if (path == NQM_SYM_DRAIN || (path == NQM_SYM_AUTO && k4_supports(n))) return k4_count(ctx, n, unique, total, stats);
+    return k3_count(ctx, n, unique, total, stats);
+}
+
+extern "C" int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
+    return nqm_count_symmetric_path(ctx, n, NQM_SYM_AUTO, unique, total, stats);
 }

@@ -152,6 +152,8 @@ _SIGNATURES = {
     "nqm_solve_batch": ([VP, ctypes.POINTER(Solve), ctypes.c_uint32, ctypes.POINTER(Tune), U64P,
                          U64P, ctypes.POINTER(Stats)], ctypes.c_int),
     "nqm_count_symmetric": ([VP, ctypes.c_uint32, U64P, U64P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_count_symmetric_path": ([VP, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, ctypes.POINTER(Stats)],
+                                 ctypes.c_int),
     "nqm_pool_count": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P],
                        ctypes.c_int),
     "nqm_pool_generate": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U8P,
@@ -382,11 +384,22 @@ def _solve_batch(ctx, solves, tune=None, raw=False, **tune_kw):
 Context.solve_batch = _solve_batch
 
 
-def _count_symmetric(ctx, n):
+SYM_AUTO = 0
+SYM_THREAD = 1  # one thread per host item (any n)
+SYM_DRAIN = 2   # the persistent drain (10 <= n <= 21)
+
+
+def _count_symmetric(ctx, n, path=None):
     """Symmetry-reduced count of the empty n x n board: (unique classes,
-    total solutions, Stats)."""
+    total solutions, Stats). path: SYM_THREAD / SYM_DRAIN / SYM_AUTO (None:
+    nqm_count_symmetric itself)."""
     u, t, st = ctypes.c_uint64(), ctypes.c_uint64(), Stats()
-    ctx._check(ctx.lib.nqm_count_symmetric(ctx.handle, n, ctypes.byref(u), ctypes.byref(t), ctypes.byref(st)))
+    if path is None:
+        rc = ctx.lib.nqm_count_symmetric(ctx.handle, n, ctypes.byref(u), ctypes.byref(t), ctypes.byref(st))
+    else:
+        rc = ctx.lib.nqm_count_symmetric_path(ctx.handle, n, path, ctypes.byref(u), ctypes.byref(t),
+                                              ctypes.byref(st))
+    ctx._check(rc)
     return u.value, t.value, st
 
 

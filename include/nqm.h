@@ -287,6 +287,18 @@ int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
  * stats->nodes = those placements (not comparable with the other paths). */
 int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats);
 
+/* The same count by a chosen kernel: NQM_SYM_THREAD = one thread per host
+ * item (any n), NQM_SYM_DRAIN = the persistent drain (10 <= n <= 21, else
+ * NQM_EUNSUPPORTED), NQM_SYM_AUTO = the drain where it applies (what
+ * nqm_count_symmetric does). Every path gives the same unique, total and
+ * stats->nodes. stats->search_ms = the drain (or thread) kernel alone,
+ * kernel_ms = from the first expansion to the last result. */
+#define NQM_SYM_AUTO 0u
+#define NQM_SYM_THREAD 1u
+#define NQM_SYM_DRAIN 2u
+int nqm_count_symmetric_path(nqm_ctx* ctx, uint32_t n, uint32_t path, uint64_t* unique, uint64_t* total,
+                             nqm_stats* stats);
+
 /* ---- serving: continuous batching of count requests ---------------------
  * No reference counterpart. Requests join a device-resident work queue
  * between budgeted drain rounds (new requests are served first); pieces of
