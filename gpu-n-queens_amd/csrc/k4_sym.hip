@@ -71,10 +71,11 @@ __host__ __device__ inline uint32_t symq_top_b1(uint32_t m) { return (m >> 25) &
 
 constexpr uint32_t kPrefixRows = 12;  // rows whose columns an item keeps
 
-// The row an item is grown to: R, or b1 for a corner item whose column-1
-// rule reaches below R.
+// The row every item is grown to (the corner rule "rows < b1 avoid column
+// 1" that reaches below it is the drain's).
 __host__ __device__ inline uint32_t symq_target(uint32_t meta, uint32_t R) {
-    return (symq_corner(meta) && symq_b1(meta) > R) ? symq_b1(meta) : R;
+    (void)meta;
+    return R;
 }
 
 // The child of item `it` (at row y) for column bit b; f = 0 if it has no
@@ -269,10 +270,12 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
 //          the item one) leaves the solution mask and is written to the
 //          wave's chunk at index cnt + rank; the others add to sols. Push
 //          as in K2. The child: ld' = (ld + b) << 1, rd' = (rd | b) >> 1,
-//          t1 = ld' >> 31 (arithmetic: the row-b2 marker), t2 = bit n of rd'
-//          (the last-row marker), avail' = (avail ^ b) & ~(lm & t2), and at
-//          row b2 the side rule from the open side columns of avail':
-//          m0 / mt = bit 0 / n-1 of avail' as 0 or ~0, blocked = side
+//          t1 = ld' >> 31 (arithmetic: the row-b2 marker, or the row-b1
+//          marker of a corner item), t2 = bit n of rd' (the last-row
+//          marker), avail' = ((avail ^ b) & ~(lm & t2)) | (amask & t1) (a
+//          corner item gets column 1 back at row b1), and at row b2 the side
+//          rule from the open side columns of avail' & sidel (sidel = 0 for
+//          a corner item): m0 / mt = bit 0 / n-1 as 0 or ~0, blocked = side
 //          columns where both are open, all others where either is; the
 //          free set is avail' & ~ld' & ~(rd' | (blocked & t1)).
 // bitop3 truth tables index (S0 S1 S2) as a 3-bit number, S0 the high bit:
@@ -314,8 +317,10 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_bfe_i32 %[t2v], " NR ", %[sn], 1\n"                 \
     "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
     "v_bitop3_b32 " NA ", " CA ", %[b], %[t2v] bitop3:0x14\n" \
-    "v_bfe_i32 %[m0], " NA ", 0, 1\n"                      \
-    "v_bfe_i32 %[mt], " NA ", %[snm1], 1\n"                \
+    "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"      \
+    "v_and_b32 %[m0], " NA ", %[sidel]\n"                 \
+    "v_bfe_i32 %[mt], %[m0], %[snm1], 1\n"                \
+    "v_bfe_i32 %[m0], %[m0], 0, 1\n"                      \
     "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
     "v_bitop3_b32 %[t1], " NR ", %[mt], %[t1] bitop3:0xf8\n" \
     "v_bitop3_b32 " NX ", " NA ", " NL ", %[t1] bitop3:0x10\n" \
@@ -343,7 +348,8 @@ typedef uint32_t k4_v2 __attribute__((ext_vector_type(2)));
 // or the chunk holds kRecLimit records (checked every 8 passes). Returns the
 // non-tie full boards (psols) and the placements (pnodes) of the wave.
 __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, uint32_t bitb1, uint32_t endb,
-                                         uint32_t item, uint32_t& cnt, uint64_t recbase, uint32_t sn,
+                                         uint32_t amask, uint32_t sidel, uint32_t item, uint32_t& cnt,
+                                         uint64_t recbase, uint32_t sn,
                                          uint32_t snm1, uint32_t sside, uint32_t floor_live, uint32_t lim32,
                                          uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     k4_v4 s4 = {st.x, st.y, st.z, st.w};
@@ -379,7 +385,8 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, ui
           [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
           [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c)
         : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [lm] "v"(lm),
-          [bitb1] "v"(bitb1), [endb] "v"(endb), [sn] "s"(sn), [snm1] "s"(snm1), [sside] "s"(sside),
+          [bitb1] "v"(bitb1), [endb] "v"(endb), [amask] "v"(amask), [sidel] "v"(sidel), [sn] "s"(sn),
+          [snm1] "s"(snm1), [sside] "s"(sside),
           [recbase] "s"(recbase)
         : "vcc", "scc", "memory");
     (void)s5;
@@ -405,7 +412,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
 
     uint4 st = make_uint4(0u, 0u, 0u, 0u);
     uint32_t q = base;
-    uint32_t lm = 0, bitb1 = 0, endb = 0, item = 0;
+    uint32_t lm = 0, bitb1 = 0, endb = 0, amask = 0, sidel = 0, item = 0;
     auto holds = [&]() { return st.x != 0u || q != base; };
 
     // the queue: nseg segments with a head each (as in K2's drain)
@@ -440,11 +447,20 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
     auto take = [&](uint32_t idx) {
         const SymQ it = a.items[idx];
         const uint32_t r = symq_row(it.meta), b1 = symq_b1(it.meta);
-        uint32_t ld = it.ld, rd = it.rd;
+        uint32_t ld = it.ld, rd = it.rd, avail = ~it.cols & a.mask;
         lm = 0;
         bitb1 = 0;
         endb = 0;
-        if (!symq_corner(it.meta)) {
+        amask = 0;
+        sidel = 0;
+        if (symq_corner(it.meta)) {
+            if (b1 > r) {  // rows below b1 avoid column 1: out of avail until the child at row b1
+                avail &= ~2u;
+                ld |= 1u << (31u - (b1 - r));
+                amask = 2u;
+            }
+        } else {
+            sidel = a.side;
             const SymCase c = sym_case(a.n, b1, 0u);
             if (c.b2 > r) ld |= 1u << (31u - (c.b2 - r));  // reaches bit 31 at child row b2
             rd |= 1u << (a.n + (a.last - r));              // reaches bit n at the last row
@@ -452,7 +468,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
             bitb1 = 1u << b1;
             endb = (b1 < r && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
         }
-        st = make_uint4(it.f, ~it.cols & a.mask, ld, rd);
+        st = make_uint4(it.f, avail, ld, rd);
         q = base;
         item = idx;
     };
@@ -515,11 +531,14 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                     const uint32_t dl = recv ? wscratch[ir] : lane;
                     const uint32_t d_lm = __shfl(lm, dl, 64), d_bitb1 = __shfl(bitb1, dl, 64);
                     const uint32_t d_endb = __shfl(endb, dl, 64), d_item = __shfl(item, dl, 64);
+                    const uint32_t d_amask = __shfl(amask, dl, 64), d_sidel = __shfl(sidel, dl, 64);
                     if (recv) {
                         st = k4_slot_load(base - 16u * lane + 16u * dl + row);  // the donor's slot 0
                         lm = d_lm;
                         bitb1 = d_bitb1;
                         endb = d_endb;
+                        amask = d_amask;
+                        sidel = d_sidel;
                         item = d_item;
                         q = base;
                     }
@@ -556,7 +575,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                                  (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)rb);
         uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
         if (holds()) {
-            k4_phase(st, q, lm, bitb1, endb, item, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit, psols,
+            k4_phase(st, q, lm, bitb1, endb, amask, sidel, item, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit, psols,
                      pnodes, ran);
             if ((int32_t)q < 0) {  // the piece ended: its last load is void
                 q = base;
@@ -585,111 +604,188 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
 }
 
 // ---------------------------------------------------------------------------
-// ties: search the recorded boards again and run the rotation check
+// ties: search the recorded boards again, then run the rotation check
 // ---------------------------------------------------------------------------
-constexpr uint64_t kEmptyKey = ~0ull;
+// A record {item, ld} stands for the boards of the item's subtree whose last
+// row's ld is that ld and which the drain records; each such board wrote
+// the same record, so a record whose search finds m boards is one of m equal
+// records. Instead of removing copies, every copy counts its m boards and
+// the sums of records with m boards are divided by m at the end (exact: each
+// group adds m equal shares).
+constexpr uint32_t kMaxMult = 16;  // boards behind one record (more is an error)
+constexpr uint64_t kMultLcm = 720720;  // lcm(1..16): every record adds lcm / m per board, summed once
 
-__device__ __forceinline__ bool k4_claim(unsigned long long* table, uint32_t tmask, uint64_t key) {
-    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & tmask;
-    for (;;) {
-        const unsigned long long old = atomicCAS(&table[h], (unsigned long long)kEmptyKey, (unsigned long long)key);
-        if (old == kEmptyKey) return true;
-        if (old == key) return false;
-        h = (h + 1u) & tmask;
-    }
-}
+constexpr uint32_t kBoardChunk = 1024;  // boards per chunk of a wave
 
-// The board of a search again: rows below r0 from the item's prefix, the
-// others from the path (5 bits a row from r0 on).
-struct PathBoard {
-    uint64_t pre, path;
-    uint32_t r0;
-    __device__ uint32_t operator()(int r) const {
-        const uint32_t u = (uint32_t)r;
-        const uint64_t v = u < r0 ? pre >> (5u * u) : path >> (5u * (u - r0));
-        return 1u << (uint32_t)(v & 31u);
-    }
+struct K4TieArgs {
+    const uint2* recs;
+    const uint32_t* rec_counts;
+    uint32_t nslots;           // used chunks x kRecChunk
+    const SymQ* items;
+    uint32_t n;
+    uint4* boards;             // chunks of {item, slot, path lo, path hi}; chunk nbchunks is the overflow sink
+    uint32_t* board_counts;    // boards in each chunk
+    uint32_t* bchunk_head;     // chunks handed out
+    uint32_t nbchunks;
+    uint32_t* mult;            // per record slot: boards found
+    uint32_t* err;
 };
 
-// Every board of `it`'s subtree whose last-row ld equals L and which the
-// drain would have recorded: (classes, total weight) after the rotation
-// check. Rows above the last keep only the columns whose diagonal bit is in
-// L. Going back up a row rebuilds that row's cols (cols ^ b) and ld (ld >> 1
-// without b: the bit 31 this loses, and those lost going up further, end
-// above bit 31 by the last row); rd cannot be rebuilt (its bit 0, lost on
-// the way down, reaches the children after a second step up), so it is kept
-// per row with the candidates left.
-__device__ void k4_research(const SymQ& it, uint32_t L, uint64_t& uq, uint64_t& tot) {
-    const uint32_t n = symq_n(it.meta), r0 = symq_row(it.meta), b1 = symq_b1(it.meta);
-    const SymCase c = sym_case(n, b1, 0u);
-    const uint32_t endb = (b1 < r0 && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
-    const uint32_t bitb1 = 1u << b1;
-    PathBoard brd{((uint64_t)it.p1 << 32) | it.p0, 0u, r0};
-    uint32_t sf[kK4MaxRows], sr[kK4MaxRows];  // rows r0..last-1
-    uint32_t y = r0, l = it.ld, d = it.cols, r = it.rd;
-    auto keep = [&](uint32_t yy) { return yy < c.last ? (L >> (c.last - yy)) & c.mask : c.mask; };
-    uint32_t f = it.f & keep(y);
+constexpr uint32_t kTieThreads = 256;
+
+// Persistent lanes: a lane takes its next record slot when it is done with
+// its record (the slots of lane g of the grid are g, g + G, g + 2 G, ...:
+// no counter that every wave of the chip would update; a wave refills once
+// 16 of its lanes idle, so the loads of a refill are shared), and lanes of
+// a wave are not held by the longest search of the wave. Rows above the last keep
+// only the columns whose diagonal bit is in the record's ld. Going back up
+// a row rebuilds that row's cols (cols ^ b) and ld (ld >> 1 without b: the
+// bits this loses end above bit 31 by the last row); rd and the candidates
+// left are kept per row in LDS ([row][thread]). A board found is written
+// out {item, slot, path} for k4_check into the wave's board chunk (one
+// atomic per chunk); the slot's count of boards goes to mult[slot] when its
+// search ends.
+__global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
+    __shared__ uint2 stk[kK4MaxRows * kTieThreads];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t n = a.n, last = n - 1u, mask = (1u << n) - 1u;
+    const uint32_t gsize = gridDim.x * kTieThreads;
+    uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next slot
+    bool act = false;
+    uint32_t slot = 0, item = 0, r0 = 0, y = 0, l = 0, d = 0, r = 0, f = 0, L = 0, m = 0;
+    uint32_t endb = 0, bitb1 = 0;
+    uint64_t path = 0;
+    SymCase c = sym_case(n, 1u, 0u);
+    auto keep = [&](uint32_t yy) { return yy < last ? (L >> (last - yy)) & mask : mask; };
+    constexpr uint32_t kNoChunk = 0xffffffffu;
+    uint32_t bchunk = kNoChunk, bcnt = 0;  // wave-uniform
     for (;;) {
-        if (f == 0u) {
-            if (y == r0) break;
-            --y;
-            const uint32_t b = 1u << (uint32_t)((brd.path >> (5u * (y - r0))) & 31u);
-            d ^= b;
-            l = (l >> 1) & ~b;
-            r = sr[y - r0];
-            f = sf[y - r0];
-            continue;
-        }
-        const uint32_t bit = f & (0u - f);
-        f ^= bit;
-        const uint32_t sh = 5u * (y - r0);
-        brd.path = (brd.path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
-        if (y == c.last) {
-            if (l == L && ((bit & endb) | (l & bitb1)) != 0u) {
-                const uint32_t w = sym_check(c, brd);
-                uq += w != 0u;
-                tot += w;
+        const uint64_t idle = __ballot(!act);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        const bool exhausted = __ballot(next < a.nslots) == 0;  // no lane has slots left
+        if (!exhausted && (nidle >= 16u || nidle == 64u)) {
+            const uint32_t s = next;
+            const bool mine = !act && s < a.nslots;
+            if (mine) next += gsize;
+            if (mine && (s % kRecChunk) < a.rec_counts[s / kRecChunk]) {
+                const uint2 rc = a.recs[s];
+                const SymQ it = a.items[rc.x];
+                slot = s;
+                item = rc.x;
+                L = rc.y;
+                r0 = symq_row(it.meta);
+                const uint32_t b1 = symq_b1(it.meta);
+                c = sym_case(n, b1, 0u);
+                endb = (b1 < r0 && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
+                bitb1 = 1u << b1;
+                y = r0;
+                l = it.ld;
+                d = it.cols;
+                r = it.rd;
+                f = it.f & keep(y);
+                path = 0;
+                m = 0;
+                act = true;
             }
-            continue;
         }
-        sf[y - r0] = f;
-        sr[y - r0] = r;
-        l = (l | bit) << 1;
-        d |= bit;
-        r = (r | bit) >> 1;
-        ++y;
-        f = sym_free(c, y, l, d, r) & keep(y);
+        if (exhausted && __ballot(act) == 0) break;
+        bool found = false;
+        if (act) {
+            if (f == 0u) {
+                if (y == r0) {
+                    a.mult[slot] = m;
+                    act = false;
+                } else {
+                    --y;
+                    const uint32_t b = 1u << (uint32_t)((path >> (5u * (y - r0))) & 31u);
+                    d ^= b;
+                    l = (l >> 1) & ~b;
+                    const uint2 e = stk[(y - r0) * kTieThreads + tid];
+                    f = e.x;
+                    r = e.y;
+                }
+            } else {
+                const uint32_t bit = f & (0u - f);
+                f ^= bit;
+                const uint32_t sh = 5u * (y - r0);
+                path = (path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
+                if (y == last) {
+                    found = l == L && ((bit & endb) | (l & bitb1)) != 0u;
+                    m += found ? 1u : 0u;
+                } else {
+                    stk[(y - r0) * kTieThreads + tid] = make_uint2(f, r);
+                    l = (l | bit) << 1;
+                    d |= bit;
+                    r = (r | bit) >> 1;
+                    ++y;
+                    f = sym_free(c, y, l, d, r) & keep(y);
+                }
+            }
+        }
+        const uint64_t fm = __ballot(found);
+        if (fm) {
+            const uint32_t nb = (uint32_t)__popcll(fm);
+            if (bchunk == kNoChunk || bcnt + nb > kBoardChunk) {
+                if (bchunk != kNoChunk && lane == 0) a.board_counts[bchunk] = bcnt;
+                uint32_t ch = 0;
+                if (lane == 0) ch = atomicAdd(a.bchunk_head, 1u);
+                ch = __builtin_amdgcn_readfirstlane(__shfl(ch, 0, 64));
+                if (ch >= a.nbchunks) {  // out of chunks: the sink (the count fails)
+                    if (lane == 0) atomicOr(a.err, 128u);
+                    ch = a.nbchunks;
+                }
+                bchunk = ch;
+                bcnt = 0;
+            }
+            if (found)
+                a.boards[(size_t)bchunk * kBoardChunk + bcnt + k4_rank(fm)] =
+                    make_uint4(item, slot, (uint32_t)path, (uint32_t)(path >> 32));
+            bcnt += nb;
+        }
     }
+    if (bchunk != kNoChunk && lane == 0) a.board_counts[bchunk] = bcnt;
 }
 
-// Chunk ch holds rec_counts[ch] records; the first copy of each {item, ld}
-// (hash set) searches again.
+// One thread per board found: the rotation check (sym_check_packed on the
+// item's prefix rows and the board's path). A board behind a record with m
+// boards is counted m times (once per copy of the record), so it adds
+// lcm(1..16) / m: acc[0] = classes x lcm, acc[1] = weight x lcm.
 __global__ void __launch_bounds__(256)
-k4_ties_kernel(const uint2* __restrict__ recs, const uint32_t* __restrict__ rec_counts, uint32_t nchunks,
-               const SymQ* __restrict__ items, unsigned long long* __restrict__ table, uint32_t tmask,
-               unsigned long long* __restrict__ out) {
+k4_check_kernel(const uint4* __restrict__ boards, const uint32_t* __restrict__ board_counts,
+                const uint32_t* __restrict__ bchunk_head, uint32_t nbchunks, const uint32_t* __restrict__ mult,
+                const SymQ* __restrict__ items, uint32_t n, unsigned long long* __restrict__ acc,
+                uint32_t* __restrict__ err) {
+    const uint32_t nch = min(*bchunk_head, nbchunks);  // past the cap: err bit 128, the count is void
     uint64_t uq = 0, tot = 0;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        const uint32_t m = rec_counts[ch];
-        const uint2* const rp = recs + (size_t)ch * kRecChunk;
-        for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
-            const uint2 rc = rp[j];
-            const uint64_t key = ((uint64_t)rc.x << 32) | rc.y;
-            if (!k4_claim(table, tmask, key)) continue;
-            k4_research(items[rc.x], rc.y, uq, tot);
+    for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const uint32_t cnt = board_counts[ch];
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+        const uint4 e = boards[(size_t)ch * kBoardChunk + j];
+        const SymQ it = items[e.x];
+        const SymCase c = sym_case(n, symq_b1(it.meta), 0u);
+        // rows 0..r0-1 from the prefix, rows r0.. from the path: 5 bits a row,
+        // rows below 12 in lo, the others in hi
+        const uint32_t sh = 5u * symq_row(it.meta);
+        const uint64_t pre = ((uint64_t)it.p1 << 32) | it.p0, path = ((uint64_t)e.w << 32) | e.z;
+        const uint64_t flo = pre | (path << sh), fhi = sh ? path >> (64u - sh) : 0ull;
+        const uint64_t lo = flo & ((1ull << 60) - 1ull), hi = (flo >> 60) | (fhi << 4);
+        const uint32_t w = sym_check_packed(c, lo, hi);
+        const uint32_t m = mult[e.y];
+        if (m >= 1u && m <= kMaxMult) {
+            const uint64_t share = kMultLcm / m;  // exact: m divides the lcm
+            uq += w != 0u ? share : 0u;
+            tot += share * w;
+        } else {
+            atomicOr(err, 256u);
+        }
         }
     }
     uq = k4_wave_sum64(uq);
     tot = k4_wave_sum64(tot);
     if ((threadIdx.x & 63u) == 0u && (uq | tot)) {
-        atomicAdd(&out[0], (unsigned long long)uq);
-        atomicAdd(&out[1], (unsigned long long)tot);
+        atomicAdd(&acc[0], (unsigned long long)uq);
+        atomicAdd(&acc[1], (unsigned long long)tot);
     }
-}
-
-__global__ void k4_fill_u64_kernel(unsigned long long* p, size_t count, unsigned long long v) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
-        p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -801,17 +897,20 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
 
     int rc;
     // counters (u32 index): [0..1] level count, [2] final cursor, [3] next
-    // cursor, [8..9] u64 expansion nodes, [16] err, [32..35] u64 tie sums,
-    // [64] chunk head, [65] records written, [128..] queue heads
+    // cursor, [8..9] u64 expansion nodes, [16] err, [64] chunk head, [65]
+    // records written, [67] board chunks,
+    // [128..639] queue heads, [640..] u64 tie sums by boards per record
     if ((rc = ensure_dev(ctx, w.ctr, 4096)) != NQM_OK) return rc;
     uint32_t* const d_cnt = static_cast<uint32_t*>(w.ctr.ptr);
     unsigned long long* const d_xnodes = reinterpret_cast<unsigned long long*>(d_cnt + 8);
     uint32_t* const d_err = d_cnt + 16;
-    unsigned long long* const d_ties = reinterpret_cast<unsigned long long*>(d_cnt + 32);
     uint32_t* const d_chunk_head = d_cnt + 64;
     uint32_t* const d_rec_total = d_cnt + 65;
+    uint32_t* const d_bchunk_head = d_cnt + 67;
     uint32_t* const d_head = d_cnt + 128;
-    static_assert(128 + kHeadSegs * kHeadStride <= 1024, "heads fit the counter page");
+    unsigned long long* const d_acc = reinterpret_cast<unsigned long long*>(d_cnt + 640);
+    static_assert(128 + kHeadSegs * kHeadStride <= 640, "heads fit before the tie sums");
+    static_assert(640 * 4 + 16 <= 4096, "the tie sums fit the counter page");
     if ((rc = ensure_dev(ctx, w.items, std::max<size_t>(hq.size(), 1) * sizeof(SymQ))) != NQM_OK) return rc;
 
     const bool known = w.sizes_n == n;
@@ -907,9 +1006,9 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         if (nchunks * kRecChunk > 0xffffffffull) return fail(ctx, NQM_ERANGE, "too many tie records");
         if ((rc = ensure_dev(ctx, w.recs, (size_t)(nchunks + 1u) * kRecChunk * sizeof(uint2))) != NQM_OK) return rc;
         if ((rc = ensure_dev(ctx, w.rec_counts, (size_t)(nchunks + 1u) * 4u)) != NQM_OK) return rc;
-        NQM_HIP(ctx, hipMemsetAsync(d_chunk_head, 0, 8, st));  // chunk head, records written
+        NQM_HIP(ctx, hipMemsetAsync(d_chunk_head, 0, 16, st));  // chunk head, records, slots, boards
         NQM_HIP(ctx, hipMemsetAsync(d_head, 0, kHeadSegs * kHeadStride * 4, st));
-        NQM_HIP(ctx, hipMemsetAsync(d_ties, 0, 16, st));
+        NQM_HIP(ctx, hipMemsetAsync(d_acc, 0, 16, st));
         K4Args a;
         a.items = items;
         a.count = nfin;
@@ -947,30 +1046,54 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         }
         NQM_HIP(ctx, hipEventElapsedTime(&search_ms, ctx->ev_start, ctx->ev_stop));
         const uint32_t used = hc[0];
-        // ties: a hash set of {item, ld}, at least twice the records
-        uint32_t tbits = 10;
-        while ((1ull << tbits) < 2ull * hc[1] && tbits < 31) ++tbits;
-        if ((rc = ensure_dev(ctx, w.table, (size_t)8u << tbits)) != NQM_OK) return rc;
-        unsigned long long* const table = static_cast<unsigned long long*>(w.table.ptr);
+        // ties: every record searched again, boards checked
+        const uint64_t nslots = (uint64_t)used * kRecChunk;
+        const uint32_t tie_blocks = (uint32_t)ctx->prop.multiProcessorCount * 8u;
+        // boards: about one per record; every research wave holds a chunk
+        const uint32_t nbchunks = 2u * (hc[1] / kBoardChunk) + tie_blocks * (kTieThreads / 64u) + 64u;
+        if ((rc = ensure_dev(ctx, w.mult, (size_t)std::max<uint64_t>(nslots, 1) * 4u)) != NQM_OK) return rc;
+        if ((rc = ensure_dev(ctx, w.boards, ((size_t)nbchunks + 1u) * kBoardChunk * sizeof(uint4) +
+                                                ((size_t)nbchunks + 1u) * 4u)) != NQM_OK)
+            return rc;
+        uint4* const d_boards = static_cast<uint4*>(w.boards.ptr);
+        uint32_t* const d_board_counts = reinterpret_cast<uint32_t*>(d_boards + ((size_t)nbchunks + 1u) * kBoardChunk);
         if (hc[1]) {
-            hipLaunchKernelGGL(k4_fill_u64_kernel, dim3(2048), dim3(256), 0, st, table, (size_t)1 << tbits,
-                               (unsigned long long)kEmptyKey);
+            K4TieArgs ta;
+            ta.recs = static_cast<const uint2*>(w.recs.ptr);
+            ta.rec_counts = static_cast<const uint32_t*>(w.rec_counts.ptr);
+            ta.nslots = (uint32_t)nslots;
+            ta.items = items;
+            ta.n = n;
+                ta.boards = d_boards;
+            ta.board_counts = d_board_counts;
+            ta.bchunk_head = d_bchunk_head;
+            ta.nbchunks = nbchunks;
+            ta.mult = static_cast<uint32_t*>(w.mult.ptr);
+            ta.err = d_err;
+            hipLaunchKernelGGL(k4_research_kernel, dim3(tie_blocks), dim3(kTieThreads), 0, st, ta);
             NQM_HIP(ctx, hipGetLastError());
-            const uint32_t grid = std::min<uint32_t>(used, 16384u);
-            hipLaunchKernelGGL(k4_ties_kernel, dim3(grid), dim3(256), 0, st, static_cast<const uint2*>(w.recs.ptr),
-                               static_cast<const uint32_t*>(w.rec_counts.ptr), used, items, table,
-                               (uint32_t)((1ull << tbits) - 1u), d_ties);
+            hipLaunchKernelGGL(k4_check_kernel, dim3((uint32_t)ctx->prop.multiProcessorCount * 4u), dim3(256), 0, st,
+                               d_boards, d_board_counts,
+                               d_bchunk_head, nbchunks, static_cast<const uint32_t*>(w.mult.ptr), items, n, d_acc,
+                               d_err);
             NQM_HIP(ctx, hipGetLastError());
         }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_x1, st));
         std::vector<uint64_t> wp((size_t)nwaves * 2u);
-        uint64_t hs[2] = {0, 0};
+        uint64_t hacc[2] = {};
         unsigned long long hx = 0;
+        uint32_t err2 = 0;
         NQM_HIP(ctx, hipMemcpyAsync(wp.data(), w.wave.ptr, wp.size() * 8u, hipMemcpyDeviceToHost, st));
-        NQM_HIP(ctx, hipMemcpyAsync(hs, d_ties, 16, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(hacc, d_acc, sizeof(hacc), hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipMemcpyAsync(&hx, d_xnodes, 8, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(&err2, d_err, 4, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
         NQM_HIP(ctx, hipEventElapsedTime(&all_ms, ctx->ev_x0, ctx->ev_x1));
+        if (err2 & 128u) return fail(ctx, NQM_ERANGE, "tie boards exceed %u chunks", nbchunks);
+        if (err2 & 256u) return fail(ctx, NQM_ERANGE, "more than %u tie boards behind one record", kMaxMult);
+        // every board was counted lcm / m times m (its m records)
+        if (hacc[0] % kMultLcm || hacc[1] % kMultLcm) return fail(ctx, NQM_EDEVICE, "tie sums not whole");
+        const uint64_t hs[2] = {hacc[0] / kMultLcm, hacc[1] / kMultLcm};
         for (uint32_t wv = 0; wv < nwaves; ++wv) {
             leaves += wp[2u * wv];
             dnodes += wp[2u * wv + 1u];

@@ -220,7 +220,7 @@ void nqm_destroy(nqm_ctx* ctx) {
         free_dev(*d);
     nqm::SymWork& sw = ctx->sym;
     for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.recs, &sw.rec_counts,
-                           &sw.table, &sw.wave})
+                           &sw.boards, &sw.mult, &sw.wave})
         free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);

@@ -97,4 +97,52 @@ __host__ __device__ inline uint32_t sym_check(const SymCase& c, const Board& b) 
     return 8u;
 }
 
+// sym_check on a board packed as 5-bit columns (row r at bit 5 r of lo for
+// r < 12, at bit 5 (r - 12) of hi above), with the same result. Each
+// rotation's row is read from the inverse permutation (column -> row)
+// instead of searched for: row i of the 90-degree rotation has its queen in
+// column last - row(column i), of the 180-degree one in column
+// last - column(last - i), of the 270-degree one in column row(column
+// last - i). The comparisons stop where the loops of sym_check stop.
+__host__ __device__ inline uint32_t sym_check_packed(const SymCase& c, uint64_t lo, uint64_t hi) {
+    auto field = [](uint64_t a, uint64_t b, uint32_t k) -> uint32_t {
+        return (uint32_t)((k < 12u ? a >> (5u * k) : b >> (5u * (k - 12u))) & 31u);
+    };
+    const uint32_t last = c.last;
+    uint64_t ilo = 0, ihi = 0;  // row of each column, packed the same way
+    for (uint32_t r = 0; r <= last; ++r) {
+        const uint32_t k = field(lo, hi, r);
+        if (k < 12u)
+            ilo |= (uint64_t)r << (5u * k);
+        else
+            ihi |= (uint64_t)r << (5u * (k - 12u));
+    }
+    if (field(lo, hi, c.b2) == 0u) {  // 90 degrees
+        uint32_t own = 1;
+        for (; own <= last; ++own) {
+            const uint32_t a = field(lo, hi, own), r = last - field(ilo, ihi, own);
+            if (a > r) return 0u;
+            if (a < r) break;
+        }
+        if (own > last) return 2u;
+    }
+    if (field(lo, hi, last) == c.b2) {  // 180 degrees (endbit is column b2)
+        uint32_t own = 1;
+        for (; own <= last; ++own) {
+            const uint32_t a = field(lo, hi, own), r = last - field(lo, hi, last - own);
+            if (a > r) return 0u;
+            if (a < r) break;
+        }
+        if (own > last) return 4u;
+    }
+    if (field(lo, hi, c.b1) == last) {  // 270 degrees
+        for (uint32_t own = 1; own <= last; ++own) {
+            const uint32_t a = field(lo, hi, own), r = field(ilo, ihi, last - own);
+            if (a > r) return 0u;
+            if (a < r) break;
+        }
+    }
+    return 8u;
+}
+
 }  // namespace nqm
