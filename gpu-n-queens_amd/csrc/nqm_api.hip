@@ -222,6 +222,8 @@ void nqm_destroy(nqm_ctx* ctx) {
     for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.recs, &sw.rec_counts,
                            &sw.boards, &sw.mult, &sw.wave})
         free_dev(*d);
+    nqm::FsWork& fw = ctx->fs;
+    for (nqm::DevBuf* d : {&fw.ctr, &fw.lvl[0], &fw.lvl[1], &fw.wave}) free_dev(*d);
     free_host(ctx->h_part);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

@@ -64,6 +64,12 @@ struct SymWork {
     std::vector<uint32_t> level_in, level_fin;    // per level: pending items in, final items out
     uint64_t chunks = 0;                          // record chunks the last count of sizes_n used
 };
+
+// Device buffers of the full-sweep count (k5_fullsweep.hip), kept across
+// calls (they only grow): counters, the two level buffers, per-workgroup sums.
+struct FsWork {
+    DevBuf ctr, lvl[2], wave;
+};
 }  // namespace nqm
 
 struct nqm_ctx {
@@ -77,6 +83,7 @@ struct nqm_ctx {
     nqm::DevBuf pool, part, err;
     nqm::BatchWork batch;
     nqm::SymWork sym;
+    nqm::FsWork fs;
     nqm::HostBuf h_part;
     std::string last_error;
 };

@@ -154,6 +154,9 @@ _SIGNATURES = {
     "nqm_count_symmetric": ([VP, ctypes.c_uint32, U64P, U64P, ctypes.POINTER(Stats)], ctypes.c_int),
     "nqm_count_symmetric_path": ([VP, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P, ctypes.POINTER(Stats)],
                                  ctypes.c_int),
+    "nqm_count_fullsweep": ([VP, ctypes.c_uint32, U64P, ctypes.POINTER(Stats)], ctypes.c_int),
+    "nqm_count_fullsweep_ex": ([VP, ctypes.c_uint32, ctypes.c_uint32, U64P, ctypes.POINTER(Stats)],
+                               ctypes.c_int),
     "nqm_pool_count": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U64P, U64P],
                        ctypes.c_int),
     "nqm_pool_generate": ([VP, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U8P,
@@ -404,6 +407,23 @@ def _count_symmetric(ctx, n, path=None):
 
 
 Context.count_symmetric = _count_symmetric
+
+
+def _count_fullsweep(ctx, n, target_states=None):
+    """Full DoubleSweep count of the empty n x n board (propagation over every
+    open row and column, centre-out branching): (total, Stats). stats.nodes
+    are the placements of this search. target_states: where the device
+    expansion stops (None: nqm_count_fullsweep's default)."""
+    t, st = ctypes.c_uint64(), Stats()
+    if target_states is None:
+        rc = ctx.lib.nqm_count_fullsweep(ctx.handle, n, ctypes.byref(t), ctypes.byref(st))
+    else:
+        rc = ctx.lib.nqm_count_fullsweep_ex(ctx.handle, n, target_states, ctypes.byref(t), ctypes.byref(st))
+    ctx._check(rc)
+    return t.value, st
+
+
+Context.count_fullsweep = _count_fullsweep
 
 
 def _pool_count(ctx, n, lock, flags=0, per_col=False):

@@ -299,6 +299,26 @@ int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* to
 int nqm_count_symmetric_path(nqm_ctx* ctx, uint32_t n, uint32_t path, uint64_t* unique, uint64_t* total,
                              nqm_stats* stats);
 
+/* ---- full DoubleSweep counting ---------------------------------------------
+ * The reference counts with DoubleSweep-light (unit propagation of the current
+ * row, GPU_NQueens.cu:392-408) and names the full algorithm as future work
+ * (README.md:6, :22, :40-41); its unbuilt attempt walks the rows centre-out
+ * (GPU_NQueens.cu:207, :214-336). This search propagates over every open row
+ * and column until no line is forced (a line with no free cell ends the
+ * branch, one with a single free cell gets its queen) and branches on the
+ * open row nearest the centre (m, m+1, m-1, ...; m = (n-1)/2), the root
+ * mirror-halved. total = the reference's count; stats->nodes = this search's
+ * placements (children and propagated queens), about 4x fewer than the light
+ * search's at n = 15..18. 1 <= n <= 21 (else NQM_EUNSUPPORTED).
+ * kernel_ms = from the first expansion level to the end of the drain,
+ * search_ms = the drain alone. */
+int nqm_count_fullsweep(nqm_ctx* ctx, uint32_t n, uint64_t* total, nqm_stats* stats);
+/* The same with the number of states the device expansion stops at (the
+ * first level holding at least target_states; 0 = 2^20). Any target gives the
+ * same total and stats->nodes. */
+int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_states, uint64_t* total,
+                           nqm_stats* stats);
+
 /* ---- serving: continuous batching of count requests ---------------------
  * No reference counterpart. Requests join a device-resident work queue
  * between budgeted drain rounds (new requests are served first); pieces of

@@ -54,6 +54,15 @@ def lib():
         L.orc_sym_count.restype = ctypes.c_int
         L.orc_sym_brute.argtypes = [ctypes.c_uint, U64P, U64P]
         L.orc_sym_brute.restype = ctypes.c_int
+        L.orc_fullsweep_count.argtypes = [ctypes.c_uint, ctypes.c_uint, U64P, U64P]
+        L.orc_fullsweep_count.restype = ctypes.c_int
+        L.orc_fullsweep_subtree.argtypes = [ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_uint64, ctypes.c_uint64, U64P, U64P]
+        L.orc_fullsweep_subtree.restype = ctypes.c_int
+        L.orc_fullsweep_children.argtypes = [ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_uint64, ctypes.c_uint64, U64P, ctypes.c_uint,
+                                             U64P]
+        L.orc_fullsweep_children.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -149,6 +158,36 @@ def sym_count(n):
     if rc != 0:
         raise ValueError(f"orc_sym_count({n}) = {rc}")
     return u.value, t.value, nd.value
+
+
+def fullsweep_count(n, threads=1):
+    """Full DoubleSweep count (nq_fullsweep.c): (solutions, placements) of the
+    mirror-halved search."""
+    s, d = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().orc_fullsweep_count(n, threads, ctypes.byref(s), ctypes.byref(d))
+    if rc != 0:
+        raise ValueError(f"orc_fullsweep_count({n}) = {rc}")
+    return s.value, d.value
+
+
+def fullsweep_subtree(n, state):
+    """(solutions, placements) below one state (P, C, D1, D2), weight 1."""
+    s, d = ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib().orc_fullsweep_subtree(n, *state, ctypes.byref(s), ctypes.byref(d))
+    if rc != 0:
+        raise ValueError(f"orc_fullsweep_subtree({n}) = {rc}")
+    return s.value, d.value
+
+
+def fullsweep_children(n, state, cap=32):
+    """(children as (P, C, D1, D2) tuples, propagated placements) of one state."""
+    out = np.zeros(4 * cap, dtype=np.uint64)
+    d = ctypes.c_uint64()
+    k = lib().orc_fullsweep_children(n, *state, out.ctypes.data_as(U64P), cap, ctypes.byref(d))
+    if k < 0:
+        raise ValueError(f"orc_fullsweep_children({n}) = {k}")
+    rows = out[: 4 * min(k, cap)].reshape(-1, 4)
+    return [tuple(int(v) for v in r) for r in rows], d.value
 
 
 def sym_brute(n):

@@ -26,6 +26,10 @@
  *       beyond these replays).
  *   orc_nqsf_* : byte layout from the format comment nq_utils.cu:127-144;
  *       no reference-written file exists (parity unpinned beyond layout).
+ *   orc_fullsweep_* (nq_fullsweep.c): solutions by test.cu:18; solutions and
+ *       nodes by an independent explicit-board restatement for small n; the
+ *       algorithm itself is defined in nq_fullsweep.c from the reference's
+ *       description (the OKlibrary source is not part of the reference).
  */
 #ifndef NQ_ORACLE_H
 #define NQ_ORACLE_H
@@ -82,6 +86,21 @@ int orc_sym_count(unsigned n, uint64_t* unique, uint64_t* total, uint64_t* nodes
 /* Every solution and its 8 images; a class is counted at its smallest member
  * (row by row). n <= 16. Returns -2 if the classes' orbits do not add up. */
 int orc_sym_brute(unsigned n, uint64_t* unique, uint64_t* total);
+
+/* Full DoubleSweep counting (nq_fullsweep.c, which defines the search):
+ * solutions and placements of the mirror-halved search of an n x n board,
+ * split over up to `threads` threads by the root's columns. */
+int orc_fullsweep_count(unsigned n, unsigned threads, uint64_t* sols, uint64_t* nodes);
+/* Solutions and placements below one state (queens already placed: rows P,
+ * columns C, diagonals D1 bit r+c and D2 bit c-r+n-1), weight 1; the state is
+ * propagated first. */
+int orc_fullsweep_subtree(unsigned n, uint32_t P, uint32_t C, uint64_t D1, uint64_t D2,
+                          uint64_t* sols, uint64_t* nodes);
+/* Propagates one state and writes its branch children (P, C, D1, D2 as four
+ * u64 each, at most cap of them); returns the number of children (0 when the
+ * state is dead or complete) and the propagated placements in *prop_nodes. */
+int orc_fullsweep_children(unsigned n, uint32_t P, uint32_t C, uint64_t D1, uint64_t D2,
+                           uint64_t* out, unsigned cap, uint64_t* prop_nodes);
 
 /* Serialize a pool in the reference's .nqsf layout (nq_utils.cu:145-193)
  * into buf (capacity cap bytes). Returns bytes written or 0 if cap is too

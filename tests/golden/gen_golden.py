@@ -14,8 +14,12 @@ Two kinds of data live here:
       n16_d5_sample.npz   -- 512 seeded depth-5 prefixes of N=16 with counts
       c3_batch.json       -- the 512-solve mixed batch (seeded, splitmix64):
                              each solve's n, prefix, count and nodes below it
+      fullsweep.json      -- full DoubleSweep search (oracle/nq_fullsweep.c):
+                             solutions and placements for N=1..18 (N >= 15 take
+                             the oracle seconds to minutes on 8 threads)
 
-Run from the repo root:  python tests/golden/gen_golden.py
+Run from the repo root:  python tests/golden/gen_golden.py [fullsweep]
+(with `fullsweep`, only fullsweep.json is written).
 """
 import json
 import os
@@ -99,7 +103,22 @@ def c3_fixture():
             "solves": rows}
 
 
+def fullsweep_fixture(max_n=18, threads=8):
+    rows = {}
+    for n in range(1, max_n + 1):
+        s, d = oracle.fullsweep_count(n, threads)
+        assert s == KNOWN[n - 1], (n, s)
+        rows[str(n)] = {"solutions": s, "nodes": d}
+        print("fullsweep", n, s, d, flush=True)
+    with open(os.path.join(HERE, "fullsweep.json"), "w") as f:
+        json.dump({"source": "oracle/nq_fullsweep.c (mirror-halved search; nodes = placements executed)",
+                   "counts": rows}, f, indent=1)
+
+
 def main():
+    if sys.argv[1:] == ["fullsweep"]:
+        fullsweep_fixture()
+        return
     with open(os.path.join(HERE, "known_totals.json"), "w") as f:
         json.dump({"source": "test.cu:18", "totals": {str(i + 1): v for i, v in enumerate(KNOWN)}},
                   f, indent=1)
@@ -125,6 +144,7 @@ def main():
                         index=idx)
     with open(os.path.join(HERE, "c3_batch.json"), "w") as f:
         json.dump(c3_fixture(), f, indent=0)
+    fullsweep_fixture()
     print("golden fixtures written")
 
 
