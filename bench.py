@@ -314,6 +314,26 @@ def symmetric_leg(ctx, nqm, n=N, reps=3):
     return out
 
 
+def fullsweep_leg(ctx, n=N, reps=3):
+    """The full DoubleSweep count of n (K5: propagation over every open row
+    and column, centre-out branching): the best of `reps` timed calls after
+    one untimed call. Its placements are its own (fewer than the light
+    search's). Raises on a wrong count."""
+    runs = []
+    for i in range(reps + 1):
+        t0 = time.perf_counter()
+        t, st = ctx.count_fullsweep(n)
+        wall = 1e3 * (time.perf_counter() - t0)
+        if t != TOTALS[n]:
+            raise RuntimeError(f"full-sweep count of {n}: {t}")
+        if i:
+            runs.append((wall, st.kernel_ms, st.search_ms, st.nodes))
+    wall, kms, sms, nodes = min(runs)
+    return {"path": "K5 full DoubleSweep (row and column propagation, centre-out branching), device "
+                    "expansion + persistent drain", "ms": wall, "kernel_ms": kms, "search_ms": sms,
+            "placements": nodes, "total": t, "placements_per_s": nodes / (sms / 1e3)}
+
+
 def workload_sym(ctx, nqm, args):
     rows = {n: symmetric_leg(ctx, nqm, n, reps=max(1, args.steps)) for n in (16, 17, 18)}
     d = rows[N]["drain"]
@@ -595,6 +615,11 @@ def main():
         # method with its own placement count, so reported beside the
         # variants, not among them
         line["symmetry_reduced"] = symmetric_leg(ctx, nqm)
+        fs = fullsweep_leg(ctx)
+        fs["light_placements_per_fullsweep_placement"] = HALF_NODES / fs["placements"]
+        if "persistent" in variants:
+            fs["ms_vs_persistent"] = fs["ms"] / variants["persistent"]["ms_per_step"]
+        line["fullsweep"] = fs
     if args.variant == "all":
         line["serving"] = serving_leg(ctx)
         if line["serving"]["mismatches"]:

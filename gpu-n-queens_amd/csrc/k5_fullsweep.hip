@@ -225,91 +225,174 @@ __device__ __forceinline__ uint32_t fs_rank(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// LDS: stack entry j of lane t at uint4 [j * 64 + t] (the state: D1 lo,
-// D1 hi | P << 9, D2 lo, D2 hi | C << 9), its untried columns at
+// ((hi:lo) >> s) & 0xffffffff for s < 32 (one v_alignbit on the device).
+__host__ __device__ __forceinline__ uint32_t fs_align(uint32_t hi, uint32_t lo, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_alignbit(hi, lo, s);
+#else
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+#endif
+}
+
+// One lane of the drain: the state of its search, advanced one step at a time
+// so that every lane of a wave does the same work per step (one line
+// evaluated, at most one pass change, branch or backtrack).
+struct FsLane {
+    uint32_t P, C, d1lo, d1hi, d2lo, d2hi;
+    uint32_t o;        // lines of the current pass not yet evaluated
+    uint32_t col;      // 1: the pass is over columns
+    uint32_t changed;  // the row or column pass placed a queen
+    uint32_t sp, w;
+    bool act;
+    uint64_t sols, nodes;
+};
+
+__host__ __device__ __forceinline__ void fs_lane_place(FsLane& L, uint32_t r, uint32_t c, uint32_t n) {
+    L.P |= 1u << r;
+    L.C |= 1u << c;
+    const uint64_t d1 = 1ull << (r + c), d2 = 1ull << (c + n - 1u - r);
+    L.d1lo |= (uint32_t)d1;
+    L.d1hi |= (uint32_t)(d1 >> 32);
+    L.d2lo |= (uint32_t)d2;
+    L.d2hi |= (uint32_t)(d2 >> 32);
+}
+
+__host__ __device__ __forceinline__ void fs_lane_start(FsLane& L, const FsItem& it, uint32_t M) {
+    L.P = it.P;
+    L.C = it.C;
+    L.d1lo = it.d1lo;
+    L.d1hi = it.d1hi;
+    L.d2lo = it.d2lo;
+    L.d2hi = it.d2hi;
+    L.w = it.w;
+    L.o = ~it.P & M;
+    L.col = 0;
+    L.changed = 0;
+    L.sp = 0;
+    L.act = true;
+}
+
+// One step of an active lane. The stack of branch points: entry j at
+// stk[j * stride] (D1 lo, D1 hi | P << 9, D2 lo, D2 hi | C << 9) and its
+// untried columns at stkf[j * stride]. Returns false when the stack would
+// overflow (the lane stops; the count is void).
+__host__ __device__ __forceinline__ bool fs_lane_step(FsLane& L, uint32_t n, uint32_t M, uint32_t stack, uint4* stk,
+                                                      uint32_t* stkf, uint32_t stride) {
+    bool dead = false;
+    if (L.o) {
+        const uint32_t line = fs_ctz(L.o);
+        L.o &= L.o - 1u;
+        const uint32_t a = fs_align(L.d1hi, L.d1lo, line);
+        uint32_t x, b;
+        if (L.col) {
+            x = L.P;
+            b = __builtin_bitreverse32(fs_align(L.d2hi, L.d2lo, line)) >> (32u - n);
+        } else {
+            x = L.C;
+            b = fs_align(L.d2hi, L.d2lo, n - 1u - line);
+        }
+        const uint32_t F = ~(x | a | b) & M;
+        if (F == 0u) {
+            dead = true;
+        } else if ((F & (F - 1u)) == 0u) {
+            const uint32_t q = fs_ctz(F);
+            fs_lane_place(L, L.col ? q : line, L.col ? line : q, n);
+            ++L.nodes;
+            L.changed = 1u;
+        }
+    }
+    bool child = false, popped = false;
+    uint32_t r = 0, f = 0, j = 0;
+    if (!dead && L.o == 0u) {
+        if (!L.col) {
+            L.col = 1u;
+            L.o = ~L.C & M;
+        } else if (L.changed) {
+            L.col = 0u;
+            L.changed = 0u;
+            L.o = ~L.P & M;
+        } else if (L.P == M) {
+            L.sols += L.w;
+            dead = true;
+        } else {
+            r = fs_branch_row(L.P, n, M);
+            f = ~(L.C | fs_align(L.d1hi, L.d1lo, r) | fs_align(L.d2hi, L.d2lo, n - 1u - r)) & M;
+            child = true;
+        }
+    }
+    if (dead) {
+        if (L.sp == 0u) {
+            L.act = false;
+        } else {
+            j = (L.sp - 1u) * stride;
+            const uint4 e = stk[j];
+            f = stkf[j];
+            const uint32_t pm = (1u << kFsPackShift) - 1u;
+            L.d1lo = e.x;
+            L.d1hi = e.y & pm;
+            L.P = e.y >> kFsPackShift;
+            L.d2lo = e.z;
+            L.d2hi = e.w & pm;
+            L.C = e.w >> kFsPackShift;
+            r = fs_branch_row(L.P, n, M);
+            child = true;
+            popped = true;
+        }
+    }
+    if (child) {
+        const uint32_t bit = f & (0u - f);
+        f ^= bit;
+        if (popped) {
+            if (f)
+                stkf[j] = f;
+            else
+                --L.sp;
+        } else if (f) {  // always: a stable open row has two free cells at least
+            if (L.sp >= stack) {
+                L.act = false;
+                return false;
+            }
+            j = L.sp * stride;
+            stk[j] = make_uint4(L.d1lo, L.d1hi | (L.P << kFsPackShift), L.d2lo, L.d2hi | (L.C << kFsPackShift));
+            stkf[j] = f;
+            ++L.sp;
+        }
+        fs_lane_place(L, r, fs_ctz(bit), n);
+        ++L.nodes;
+        L.col = 0u;
+        L.changed = 0u;
+        L.o = ~L.P & M;
+    }
+    return true;
+}
+
+// LDS: entry j of lane t at uint4 [j * 64 + t], its untried columns at
 // u32 [stack * 64 * 4 + j * 64 + t].
 __global__ void __launch_bounds__(kFsDrainThreads) fs_drain_kernel(FsDrainArgs a) {
     extern __shared__ uint4 fs_lds[];
-    uint4* stk = fs_lds;
-    uint32_t* stkf = reinterpret_cast<uint32_t*>(fs_lds + a.stack * kFsDrainThreads);
     const uint32_t lane = threadIdx.x;
+    uint4* stk = fs_lds + lane;
+    uint32_t* stkf = reinterpret_cast<uint32_t*>(fs_lds + a.stack * kFsDrainThreads) + lane;
     const uint32_t n = a.n, M = (uint32_t)((1ull << n) - 1ull);
-    const uint32_t pmask = (1u << kFsPackShift) - 1u;
-    FsState s{};
-    uint32_t w = 0, sp = 0;
-    bool act = false, drained = false;
-    uint64_t sols = 0, nodes = 0;
+    FsLane L{};
+    bool drained = false;
     for (;;) {
-        const uint64_t idle = __ballot(!act);
+        const uint64_t idle = __ballot(!L.act);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!drained && (nidle >= a.refill_idle || nidle == 64u)) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(a.head, nidle);
             base = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)base, 0, 64));
             if (base >= a.count) drained = true;
-            if (!act) {
+            if (!L.act) {
                 const uint32_t idx = base + fs_rank(idle);
-                if (idx < a.count) {
-                    const FsItem it = a.items[idx];
-                    s = fs_load(it);
-                    w = it.w;
-                    sp = 0;
-                    act = true;
-                }
+                if (idx < a.count) fs_lane_start(L, a.items[idx], M);
             }
         }
-        if (drained && __ballot(act) == 0) break;
-        if (!act) continue;
-        uint32_t pn = 0;
-        bool back = true;
-        if (fs_propagate(s, n, M, pn)) {
-            if (s.P == M) {
-                sols += w;
-            } else if (sp >= a.stack) {
-                atomicOr(a.err, 2u);
-                act = false;
-                back = false;
-            } else {
-                const uint32_t r = fs_branch_row(s.P, n, M);
-                uint32_t f = fs_row_free(s, r, n, M);
-                const uint32_t b = f & (0u - f);
-                f ^= b;
-                if (f) {
-                    const uint32_t j = sp * kFsDrainThreads + lane;
-                    stk[j] = make_uint4((uint32_t)s.D1, (uint32_t)(s.D1 >> 32) | (s.P << kFsPackShift),
-                                        (uint32_t)s.D2, (uint32_t)(s.D2 >> 32) | (s.C << kFsPackShift));
-                    stkf[j] = f;
-                    ++sp;
-                }
-                fs_place(s, r, fs_ctz(b), n);
-                ++pn;
-                back = false;
-            }
-        }
-        if (back) {
-            if (sp == 0) {
-                act = false;
-            } else {
-                const uint32_t j = (sp - 1u) * kFsDrainThreads + lane;
-                const uint4 e = stk[j];
-                uint32_t f = stkf[j];
-                s.D1 = ((uint64_t)(e.y & pmask) << 32) | e.x;
-                s.P = e.y >> kFsPackShift;
-                s.D2 = ((uint64_t)(e.w & pmask) << 32) | e.z;
-                s.C = e.w >> kFsPackShift;
-                const uint32_t r = fs_branch_row(s.P, n, M);
-                const uint32_t b = f & (0u - f);
-                f ^= b;
-                if (f)
-                    stkf[j] = f;
-                else
-                    --sp;
-                fs_place(s, r, fs_ctz(b), n);
-                ++pn;
-            }
-        }
-        nodes += pn;
+        if (drained && __ballot(L.act) == 0) break;
+        if (L.act && !fs_lane_step(L, n, M, a.stack, stk, stkf, kFsDrainThreads)) atomicOr(a.err, 2u);
     }
+    uint64_t sols = L.sols, nodes = L.nodes;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         sols += __shfl_xor(sols, o, 64);
@@ -336,7 +419,7 @@ extern "C" int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_
     const double t0 = now_ms();
     NQM_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    const uint32_t target = target_states ? target_states : (1u << 20);
+    const uint32_t target = target_states ? target_states : (1u << 22);
     FsWork& W = ctx->fs;
 
     // the root's children: row m, left-half columns weigh 2, the middle of odd n 1

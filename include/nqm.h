@@ -314,8 +314,9 @@ int nqm_count_symmetric_path(nqm_ctx* ctx, uint32_t n, uint32_t path, uint64_t* 
  * search_ms = the drain alone. */
 int nqm_count_fullsweep(nqm_ctx* ctx, uint32_t n, uint64_t* total, nqm_stats* stats);
 /* The same with the number of states the device expansion stops at (the
- * first level holding at least target_states; 0 = 2^20). Any target gives the
- * same total and stats->nodes. */
+ * first level holding at least target_states; 0 = 2^22: at n = 17, 18 the
+ * drain's waves then end within 5% of each other). Any target gives the same
+ * total and stats->nodes. */
 int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_states, uint64_t* total,
                            nqm_stats* stats);
 
