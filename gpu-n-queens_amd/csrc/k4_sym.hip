@@ -27,11 +27,11 @@
 //    top queen's corner distance: the last queen on column b2, the column-0
 //    queen on row b2 -- which puts bit b1 into the last row's ld -- or row
 //    b1's queen on column n-1) is not counted in the drain: the lane writes
-//    {item, ld of the last row} into its wave's record chunk. Every other
+//    {item, ld and rd of the last row} into its wave's record chunk. Every other
 //    full board is its class's only representative and counts 8.
 //
 //  * k4_ties: one thread per record searches the item again, keeping only
-//    the columns whose diagonal bit is in the recorded ld, and runs the
+//    the columns whose diagonal bits are in the recorded ld and rd, and runs the
 //    rotation check (sym_check) on the boards it finds. Several boards of one
 //    item can share an ld, and each of them wrote the same record: a hash
 //    set keeps the first copy, which counts them all.
@@ -221,7 +221,7 @@ struct K4Args {
     uint32_t* head;       // nseg heads, kHeadStride u32 apart
     uint32_t n, last, mask, side;
     uint32_t refill_idle;
-    uint2* recs;          // record chunks (kRecChunk records each); chunk nchunks is the overflow sink
+    uint4* recs;          // record chunks (kRecChunk records {item, ld, rd, 0} each); chunk nchunks is the overflow sink
     uint32_t* rec_counts; // records in each chunk, written when the wave closes it
     uint32_t* chunk_head; // chunks handed out
     uint32_t* rec_total;  // records written
@@ -262,7 +262,8 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
 // One search phase (asm: it manages the execution mask itself), the pass of
 // K2's drain with the row rules and the tie records added. Registers:
 // v[88:91] / v[92:95] = {f, avail, ld, rd} of the current row, alternating
-// per pass; v[96:97] = the record {item, ld} (v96 holds the lane's item).
+// per pass; v[96:99] = the record {item, ld, rd, 0} (v96 holds the lane's
+// item).
 // A pass:
 //   pop    as in K2 (v_cmpx on an empty row, borrow = piece done);
 //   step   b = f & -f; a full board when avail == b. A tie board (the last
@@ -329,9 +330,10 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "s_mov_b64 exec, %[se]\n"                              \
     "v_mbcnt_lo_u32_b32 %[e], exec_lo, 0\n"                \
     "v_mbcnt_hi_u32_b32 %[e], exec_hi, %[e]\n"             \
-    "v_add_lshl_u32 %[e], %[e], %[cnt], 3\n"               \
+    "v_add_lshl_u32 %[e], %[e], %[cnt], 4\n"               \
     "v_mov_b32 v97, " CL "\n"                              \
-    "global_store_dwordx2 %[e], v[96:97], %[recbase]\n"    \
+    "v_mov_b32 v98, " CR "\n"                              \
+    "global_store_dwordx4 %[e], v[96:99], %[recbase]\n"    \
     "s_bcnt1_i32_b64 %[t], %[se]\n"                        \
     "s_add_u32 %[cnt], %[cnt], %[t]\n"                     \
     "s_mov_b64 exec, %[alive]\n"                           \
@@ -342,7 +344,6 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
                      NQM_K4_PASS_A(4) NQM_K4_PASS_B(5) NQM_K4_PASS_A(6) NQM_K4_PASS_B(7)
 
 typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
-typedef uint32_t k4_v2 __attribute__((ext_vector_type(2)));
 
 // Runs passes until at most floor_live lanes hold a piece, lim32 passes ran
 // or the chunk holds kRecLimit records (checked every 8 passes). Returns the
@@ -354,7 +355,7 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, ui
                                          uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     k4_v4 s4 = {st.x, st.y, st.z, st.w};
     k4_v4 s5;
-    k4_v2 rec = {item, 0u};
+    k4_v4 rec = {item, 0u, 0u, 0u};
     uint64_t live, alive, sm, se;
     uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
     uint32_t sols = 0, nodes = 0;
@@ -380,7 +381,7 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, ui
         "s_add_u32 %[nodes], %[nodes], %[t]\n"
         "s_mov_b64 exec, %[live]\n"
         "s_waitcnt lgkmcnt(0)\n"
-        : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:97]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng),
+        : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:99]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng),
           [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live),
           [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
           [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c)
@@ -606,9 +607,9 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
 // ---------------------------------------------------------------------------
 // ties: search the recorded boards again, then run the rotation check
 // ---------------------------------------------------------------------------
-// A record {item, ld} stands for the boards of the item's subtree whose last
-// row's ld is that ld and which the drain records; each such board wrote
-// the same record, so a record whose search finds m boards is one of m equal
+// A record {item, ld, rd} stands for the boards of the item's subtree whose
+// last row has that ld and rd and which the drain records; each such board
+// wrote the same record, so a record whose search finds m boards is one of m equal
 // records. Instead of removing copies, every copy counts its m boards and
 // the sums of records with m boards are divided by m at the end (exact: each
 // group adds m equal shares).
@@ -618,7 +619,7 @@ constexpr uint64_t kMultLcm = 720720;  // lcm(1..16): every record adds lcm / m 
 constexpr uint32_t kBoardChunk = 1024;  // boards per chunk of a wave
 
 struct K4TieArgs {
-    const uint2* recs;
+    const uint4* recs;
     const uint32_t* rec_counts;
     uint32_t nslots;           // used chunks x kRecChunk
     const SymQ* items;
@@ -638,7 +639,9 @@ constexpr uint32_t kTieThreads = 256;
 // no counter that every wave of the chip would update; a wave refills once
 // 16 of its lanes idle, so the loads of a refill are shared), and lanes of
 // a wave are not held by the longest search of the wave. Rows above the last keep
-// only the columns whose diagonal bit is in the record's ld. Going back up
+// only the columns whose diagonal bits are in the record's ld and rd (a
+// column whose rd bit falls below bit 0 by the last row is not limited by
+// rd), which leaves few candidates per row. Going back up
 // a row rebuilds that row's cols (cols ^ b) and ld (ld >> 1 without b: the
 // bits this loses end above bit 31 by the last row); rd and the candidates
 // left are kept per row in LDS ([row][thread]). A board found is written
@@ -652,11 +655,17 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
     const uint32_t gsize = gridDim.x * kTieThreads;
     uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next slot
     bool act = false;
-    uint32_t slot = 0, item = 0, r0 = 0, y = 0, l = 0, d = 0, r = 0, f = 0, L = 0, m = 0;
+    uint32_t slot = 0, item = 0, r0 = 0, y = 0, l = 0, d = 0, r = 0, f = 0, L = 0, Rd = 0, m = 0;
     uint32_t endb = 0, bitb1 = 0;
     uint64_t path = 0;
     SymCase c = sym_case(n, 1u, 0u);
-    auto keep = [&](uint32_t yy) { return yy < last ? (L >> (last - yy)) & mask : mask; };
+    // a queen on column c of row yy < last sets ld bit c + (last - yy) and,
+    // when c >= last - yy, rd bit c - (last - yy) of the last row
+    auto keep = [&](uint32_t yy) {
+        if (yy >= last) return mask;
+        const uint32_t k = last - yy;
+        return (L >> k) & ((Rd << k) | ((1u << k) - 1u)) & mask;
+    };
     constexpr uint32_t kNoChunk = 0xffffffffu;
     uint32_t bchunk = kNoChunk, bcnt = 0;  // wave-uniform
     for (;;) {
@@ -668,11 +677,12 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
             const bool mine = !act && s < a.nslots;
             if (mine) next += gsize;
             if (mine && (s % kRecChunk) < a.rec_counts[s / kRecChunk]) {
-                const uint2 rc = a.recs[s];
+                const uint4 rc = a.recs[s];
                 const SymQ it = a.items[rc.x];
                 slot = s;
                 item = rc.x;
                 L = rc.y;
+                Rd = rc.z & mask;  // the drain's rd carries a marker above bit n-1
                 r0 = symq_row(it.meta);
                 const uint32_t b1 = symq_b1(it.meta);
                 c = sym_case(n, b1, 0u);
@@ -710,7 +720,7 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
                 const uint32_t sh = 5u * (y - r0);
                 path = (path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
                 if (y == last) {
-                    found = l == L && ((bit & endb) | (l & bitb1)) != 0u;
+                    found = l == L && r == Rd && ((bit & endb) | (l & bitb1)) != 0u;
                     m += found ? 1u : 0u;
                 } else {
                     stk[(y - r0) * kTieThreads + tid] = make_uint2(f, r);
@@ -1004,7 +1014,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
     float search_ms = 0.f, all_ms = 0.f;
     for (int attempt = 0;; ++attempt) {
         if (nchunks * kRecChunk > 0xffffffffull) return fail(ctx, NQM_ERANGE, "too many tie records");
-        if ((rc = ensure_dev(ctx, w.recs, (size_t)(nchunks + 1u) * kRecChunk * sizeof(uint2))) != NQM_OK) return rc;
+        if ((rc = ensure_dev(ctx, w.recs, (size_t)(nchunks + 1u) * kRecChunk * sizeof(uint4))) != NQM_OK) return rc;
         if ((rc = ensure_dev(ctx, w.rec_counts, (size_t)(nchunks + 1u) * 4u)) != NQM_OK) return rc;
         NQM_HIP(ctx, hipMemsetAsync(d_chunk_head, 0, 16, st));  // chunk head, records, slots, boards
         NQM_HIP(ctx, hipMemsetAsync(d_head, 0, kHeadSegs * kHeadStride * 4, st));
@@ -1019,7 +1029,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         a.mask = (1u << n) - 1u;
         a.side = (1u << last) | 1u;
         a.refill_idle = 8u;
-        a.recs = static_cast<uint2*>(w.recs.ptr);
+        a.recs = static_cast<uint4*>(w.recs.ptr);
         a.rec_counts = static_cast<uint32_t*>(w.rec_counts.ptr);
         a.chunk_head = d_chunk_head;
         a.rec_total = d_rec_total;
@@ -1059,7 +1069,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         uint32_t* const d_board_counts = reinterpret_cast<uint32_t*>(d_boards + ((size_t)nbchunks + 1u) * kBoardChunk);
         if (hc[1]) {
             K4TieArgs ta;
-            ta.recs = static_cast<const uint2*>(w.recs.ptr);
+            ta.recs = static_cast<const uint4*>(w.recs.ptr);
             ta.rec_counts = static_cast<const uint32_t*>(w.rec_counts.ptr);
             ta.nslots = (uint32_t)nslots;
             ta.items = items;
