@@ -22,7 +22,8 @@
 //    bit n at the last row (rd only moves down, and never below n). Both ride
 //    along in the stack slots and in donated rows like the real bits. The
 //    item's ld is cut to n bits, so real diagonal bits stay below the markers
-//    for n <= 21.
+//    for n <= 21; the stack of last - R rows fits LDS at 16 waves per CU for
+//    n <= 19 (k4_supports).
 //    A full board whose class may have another representative (a tie of the
 //    top queen's corner distance: the last queen on column b2, the column-0
 //    queen on row b2 -- which puts bit b1 into the last row's ld -- or row
@@ -865,7 +866,7 @@ static int grow_keep(nqm_ctx* ctx, DevBuf& b, size_t used, size_t bytes) {
 
 int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
     const double t0 = now_ms();
-    if (!k4_supports(n)) return fail(ctx, NQM_EUNSUPPORTED, "the symmetric drain takes 10 <= n <= 21, not %u", n);
+    if (!k4_supports(n)) return fail(ctx, NQM_EUNSUPPORTED, "the symmetric drain takes 10 <= n <= 19, not %u", n);
     NQM_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     SymWork& w = ctx->sym;

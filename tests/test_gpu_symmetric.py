@@ -55,6 +55,6 @@ def test_drain_n17_n18(ctx, known_totals):
 
 
 def test_drain_rejects_sizes_outside_its_range(ctx):
-    for n in (9, 22):
+    for n in (9, 20, 21, 22):
         rc = ctx.lib.nqm_count_symmetric_path(ctx.handle, n, nqm.SYM_DRAIN, None, None, None)
         assert rc == nqm.EUNSUPPORTED, n
