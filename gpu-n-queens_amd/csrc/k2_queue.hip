@@ -380,6 +380,7 @@ struct DrainArgs {
     uint32_t* err;
     uint32_t stack_rows;
     uint32_t refill_idle;
+    uint32_t next_refill;  // a phase ends once this many lanes started their next item
     uint32_t donate;
     uint32_t cont_cap;
     uint64_t budget;  // 0: drain everything
@@ -470,7 +471,8 @@ constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per a
 // asm block because it manages the execution mask itself: a lane whose piece
 // is done leaves the mask at once (it issues nothing more), and the phase
 // ends at a wave-uniform check every 8 passes, once at most floor_live
-// lanes are left or lim32 passes ran. In C++ the compiler's lowering of this
+// lanes are left, at most nfloor lanes still hold an unstarted next item
+// (signed: -1 never ends it) or lim32 passes ran. In C++ the compiler's lowering of this
 // control flow spent 4-7 extra instructions per pass on copies and mask
 // merges. Here a pass is 13 VALU (2 for the pop, 11 for the step), 5 SALU
 // (with a wave-counted solution sum), one branch, one s_waitcnt and at most
@@ -491,9 +493,12 @@ constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per a
 // (the b128 tuple of the pop and the push), alternating per pass; q = the
 // lane's stack register (layout above); alive = the lanes still placing. A
 // pass:
-//   pop    lanes whose row has no candidate left: q -= 16 T (the borrow marks
-//          a finished piece, which leaves alive; its placements, the pass
-//          index, are added to nodes) and load the slot at q + 16 T;
+//   pop    lanes whose row has no candidate left: q -= 16 T and load the
+//          slot at q + 16 T. The borrow marks a finished piece: a lane in
+//          hasnext (it holds its next queue item, of its own solve and
+//          weight, in v[96:99]) starts that item in this pass (q back to its
+//          base, the bit leaves hasnext); any other leaves alive, and its
+//          placements, the pass index, are added to nodes;
 //   step   b = f & -f; a solution when avail == b (every column taken);
 //          lanes with candidates left push {f ^ b, avail, ld, rd} at
 //          q + 16 T and move q up; the child into the other set:
@@ -511,11 +516,22 @@ constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per a
 // kWaveSols (every lane of one solve and weight): *psols and *pnodes are
 // the wave's solutions and placements, summed from lane masks in scalar
 // registers; else each lane's own. *ran = passes of the phase.
-#define NQM_K2_POP(U, CX, CT)                              \
+#define NQM_K2_POP(U, CX, CA, CL, CR, CT)                  \
     "v_cmpx_eq_u32 vcc, 0, " CX "\n"                       \
     "v_subrev_co_u32 %[q], vcc, %[row], %[q]\n"            \
     "ds_read_b128 " CT ", %[q] offset:%[row]\n"            \
     "s_cbranch_vccz .Lk2_nd" #U "_%=\n"                    \
+    "s_and_b64 exec, vcc, %[hasnext]\n"                    \
+    "s_cbranch_execz .Lk2_nn" #U "_%=\n"                   \
+    "s_waitcnt lgkmcnt(0)\n"                               \
+    "v_mov_b32 " CX ", v96\n"                              \
+    "v_mov_b32 " CA ", v97\n"                              \
+    "v_mov_b32 " CL ", v98\n"                              \
+    "v_mov_b32 " CR ", v99\n"                              \
+    "v_add_u32 %[q], %[row], %[q]\n"                       \
+    "s_andn2_b64 %[hasnext], %[hasnext], exec\n"           \
+    "s_andn2_b64 vcc, vcc, exec\n"                         \
+    ".Lk2_nn" #U "_%=:\n"                                  \
     "s_andn2_b64 %[alive], %[alive], vcc\n"                \
     "s_bcnt1_i32_b64 %[t], vcc\n"                          \
     "s_add_u32 %[t2], %[ran], " #U "\n"                    \
@@ -544,8 +560,8 @@ constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per a
     "s_add_u32 %[sols], %[sols], %[t]\n"
 #define NQM_K2_SOLS_LANE                                   \
     "v_addc_co_u32 %[sols], vcc, 0, %[sols], %[sm]\n"
-#define NQM_K2_PASS_A(U, SC) NQM_K2_POP(U, "v88", "v[88:91]") NQM_K2_STEP("v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", SC)
-#define NQM_K2_PASS_B(U, SC) NQM_K2_POP(U, "v92", "v[92:95]") NQM_K2_STEP("v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", SC)
+#define NQM_K2_PASS_A(U, SC) NQM_K2_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") NQM_K2_STEP("v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", SC)
+#define NQM_K2_PASS_B(U, SC) NQM_K2_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") NQM_K2_STEP("v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", SC)
 #define NQM_K2_PASS_WAVE(U) NQM_K2_PASS_##U(NQM_K2_SOLS_WAVE)
 #define NQM_K2_PASS_LANE(U) NQM_K2_PASS_##U(NQM_K2_SOLS_LANE) "v_add_u32 %[lnodes], 1, %[lnodes]\n"
 #define NQM_K2_PASS_0(SC) NQM_K2_PASS_A(0, SC)
@@ -566,6 +582,9 @@ constexpr uint32_t kGrabMax = kGrabItems;    // queue items one wave takes per a
     "s_bcnt1_i32_b64 %[t], exec\n"                         \
     "s_cmp_le_u32 %[t], %[floor]\n"                        \
     "s_cbranch_scc1 .Lk2_end_%=\n"                         \
+    "s_bcnt1_i32_b64 %[t], %[hasnext]\n"                   \
+    "s_cmp_le_i32 %[t], %[nfloor]\n"                       \
+    "s_cbranch_scc1 .Lk2_end_%=\n"                         \
     "s_cmp_lt_u32 %[ran], %[lim]\n"                        \
     "s_cbranch_scc1 .Lk2_pass_%=\n"                        \
     ".Lk2_end_%=:\n" END                                   \
@@ -576,10 +595,12 @@ static_assert(NQM_PASS_UNROLL == 8, "the asm phase votes every 8 passes");
 typedef uint32_t v4u_t __attribute__((ext_vector_type(4)));
 
 template <uint32_t T, bool kWaveSols>
-__device__ __forceinline__ void search_phase(uint4& st, uint32_t& q, uint32_t floor_live, uint32_t lim32,
+__device__ __forceinline__ void search_phase(uint4& st, uint32_t& q, const uint4& nx, uint64_t& hasnext,
+                                             int32_t nfloor, uint32_t floor_live, uint32_t lim32,
                                              uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     v4u_t s4 = {st.x, st.y, st.z, st.w};
     v4u_t s5;  // the other register set (scratch outside the block)
+    const v4u_t n4 = {nx.x, nx.y, nx.z, nx.w};
     uint64_t live, alive, sm;
     uint32_t b, n, t, t2;
     uint32_t sols = 0, nodes = 0;
@@ -591,8 +612,9 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& q, uint32_t fl
                                                             "s_add_u32 %[nodes], %[nodes], %[t]\n")
                      : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), [q] "+v"(q), [b] "=&v"(b), [n] "=&v"(n),
                        [live] "=&s"(live), [alive] "=&s"(alive), [sm] "=&s"(sm), [t] "=&s"(t), [t2] "=&s"(t2),
-                       [ran] "=&s"(ran), [sols] "=&s"(sols), [nodes] "=&s"(nodes)
-                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32)
+                       [ran] "=&s"(ran), [sols] "=&s"(sols), [nodes] "=&s"(nodes), [hasnext] "+s"(hasnext)
+                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32), [nfloor] "s"(nfloor),
+                       "{v[96:99]}"(n4)
                      : "vcc", "scc", "memory");
         psols = sols;
         pnodes = nodes;
@@ -603,8 +625,10 @@ __device__ __forceinline__ void search_phase(uint4& st, uint32_t& q, uint32_t fl
         asm volatile(NQM_K2_PHASE(NQM_K2_PASS_LANE, "")
                      : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), [q] "+v"(q), [b] "=&v"(b), [n] "=&v"(n),
                        [live] "=&s"(live), [alive] "=&s"(alive), [sm] "=&s"(sm), [t] "=&s"(t), [t2] "=&s"(t2),
-                       [ran] "=&s"(ran), [sols] "+v"(lsols), [nodes] "=&s"(nodes), [lnodes] "+v"(lnodes)
-                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32)
+                       [ran] "=&s"(ran), [sols] "+v"(lsols), [nodes] "=&s"(nodes), [lnodes] "+v"(lnodes),
+                       [hasnext] "+s"(hasnext)
+                     : [row] "i"(Stack<T>::kRow), [floor] "s"(floor_live), [lim] "s"(lim32), [nfloor] "s"(nfloor),
+                       "{v[96:99]}"(n4)
                      : "vcc", "scc", "memory");
         psols = lsols;
         pnodes = lnodes;
@@ -656,6 +680,17 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
     const uint32_t cont_count = a.cont_in ? *a.cont_in_count : 0u;
     bool cont_open = cont_count != 0;
     uint64_t pass = 0;  // wave-uniform: passes run so far
+
+    // Next item (single-launch drains): a lane that holds a piece also holds
+    // the queue item it starts when the piece ends. When that item has the
+    // lane's solve and weight (bit set in hasnext), the search phase starts
+    // it at once, so the lane does not idle until the phase ends; otherwise
+    // the lane starts it at the next refill.
+    const bool prefetch = a.budget == 0;
+    uint4 nx = make_uint4(0u, 0u, 0u, 0u);  // {f, avail, ld, rd} of the next item
+    uint32_t nsolve = 0, nws = 0, nmask = 0;
+    bool holdnext = false;
+    uint64_t hasnext = 0;  // wave-uniform: lanes whose next item the phase may start
 
     // resume parked pieces exactly where they stopped: current row in VGPRs,
     // open rows to LDS
@@ -748,40 +783,69 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         // ---- refill idle lanes: parked pieces of earlier rounds, then held
         // items, then the global queue (one atomic per grab of up to kGrabMax
         // items), then in-wave donation
+
+        // The queue items for the lanes of `want` (wave-collective): the
+        // held range first, then a new grab; returns the lane's item index
+        // or kNoItem.
+        constexpr uint32_t kNoItem = 0xffffffffu;
+        auto take_queue = [&](uint64_t want) -> uint32_t {
+            const uint32_t nw = __popcll(want);
+            const uint32_t take_local = min(nw, lq_end - lq);
+            uint32_t g = 0, got = 0;
+            if (take_local < nw && queue_open) {
+                // guided self-scheduling: a share of what is left, at
+                // least what the lanes need
+                const uint32_t need = nw - take_local;
+                uint32_t wantn = rem / (2u * seg_waves);
+                wantn = wantn < need ? need : (wantn > kGrabMax ? kGrabMax : wantn);
+                const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
+                if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, wantn);
+                g = __shfl(g, 0, 64);
+                got = g >= len ? 0u : min(wantn, len - g);
+                rem = g >= len ? 0u : len - g - got;
+                g += lo;
+                if (got < wantn) {  // this segment is used up: the next one
+                    if (++seg_done == nseg) queue_open = false;
+                    seg = seg + 1u == nseg ? 0u : seg + 1u;
+                    rem = seg_lo(seg + 1) - seg_lo(seg);
+                }
+            }
+            const uint32_t r = lane_rank(want);
+            uint32_t idx = kNoItem;
+            if ((want >> lane) & 1u) {
+                if (r < take_local)
+                    idx = lq + r;
+                else if (r - take_local < got)
+                    idx = g + (r - take_local);
+            }
+            lq += take_local;
+            if (got) {  // the held range was used up: keep the rest of the grab
+                lq = g + min(got, nw - take_local);
+                lq_end = g + got;
+            }
+            return idx;
+        };
+
+        // a lane whose piece ended and whose next item the phase could not
+        // start (another solve or weight) starts it now
+        if (prefetch) {
+            const bool use = !holds() && holdnext;
+            switch_solve(a, use && nsolve != acc_solve, nsolve, acc_solve, acc_sols, acc_nodes, lane);
+            if (use) {
+                mask = nmask;
+                st = nx;
+                solve = nsolve;
+                ws = nws;
+                q = base;
+                holdnext = false;
+            }
+        }
         uint64_t idle = __ballot(!holds());
         if (idle) {
             if (idle && cont_open && !a.items_first) resume_parked(idle);
-            const uint32_t nidle = __popcll(idle);
             if (idle && (queue_open || lq < lq_end)) {
-                const uint32_t take_local = min(nidle, lq_end - lq);
-                uint32_t g = 0, got = 0;
-                if (take_local < nidle && queue_open) {
-                    // guided self-scheduling: a share of what is left, at
-                    // least what the idle lanes need
-                    const uint32_t need = nidle - take_local;
-                    uint32_t want = rem / (2u * seg_waves);
-                    want = want < need ? need : (want > kGrabMax ? kGrabMax : want);
-                    const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
-                    if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, want);
-                    g = __shfl(g, 0, 64);
-                    got = g >= len ? 0u : min(want, len - g);
-                    rem = g >= len ? 0u : len - g - got;
-                    g += lo;
-                    if (got < want) {  // this segment is used up: the next one
-                        if (++seg_done == nseg) queue_open = false;
-                        seg = seg + 1u == nseg ? 0u : seg + 1u;
-                        rem = seg_lo(seg + 1) - seg_lo(seg);
-                    }
-                }
-                const uint32_t r = lane_rank(idle);
-                uint32_t idx = 0xffffffffu;
-                if (!holds()) {
-                    if (r < take_local)
-                        idx = lq + r;
-                    else if (r - take_local < got)
-                        idx = g + (r - take_local);
-                }
-                const bool take = idx != 0xffffffffu;
+                const uint32_t idx = take_queue(idle);
+                const bool take = idx != kNoItem;
                 Item it{};
                 if (take) it = a.items[idx];
                 switch_solve(a, take && it.solve != acc_solve, it.solve, acc_solve, acc_sols, acc_nodes, lane);
@@ -791,11 +855,6 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                     solve = it.solve;
                     ws = meta_wshift(it.meta);
                     q = base;
-                }
-                lq += take_local;
-                if (got) {  // the held range was used up: keep the rest of the grab
-                    lq = g + min(got, nidle - take_local);
-                    lq_end = g + got;
                 }
                 idle = __ballot(!holds());
             }
@@ -841,6 +900,23 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
                 }
             }
         }
+        // lanes with a piece and no next item take one
+        if (prefetch && (queue_open || lq < lq_end)) {
+            const uint64_t want = __ballot(holds() && !holdnext);
+            if (want) {
+                const uint32_t idx = take_queue(want);
+                if (idx != kNoItem) {
+                    const Item it = a.items[idx];
+                    nmask = (1u << meta_n(it.meta)) - 1u;
+                    nx = make_uint4(it.f, ~it.cols & nmask, it.ld, it.rd);
+                    nsolve = it.solve;
+                    nws = meta_wshift(it.meta);
+                    holdnext = true;
+                }
+            }
+        }
+        hasnext = __ballot(holdnext && nsolve == solve && nws == ws);
+
         const uint64_t live_mask = __ballot(holds());
         const bool have_items = queue_open || lq < lq_end || cont_open;
         if (live_mask == 0 && !have_items) break;
@@ -870,11 +946,18 @@ __global__ void __launch_bounds__(T) k2_drain_kernel(DrainArgs a) {
         const bool live = holds();
         const bool uniform = __ballot(live && (solve != lead_solve || ws != lead_ws)) == 0;
         uint32_t psols = 0, pnodes = 0, ran = 0;  // this phase (<= kPhaseCap passes: no wrap)
+        // the phase also ends once next_refill lanes have started their next
+        // item (while the queue can give them another)
+        const int32_t nfloor = __builtin_amdgcn_readfirstlane(
+            (queue_open || lq < lq_end) ? (int32_t)__popcll(hasnext) - (int32_t)a.next_refill : -1);
         if (live) {
+            const uint64_t had = hasnext;
             if (uniform)
-                search_phase<T, true>(st, q, floor_live, lim32, psols, pnodes, ran);
+                search_phase<T, true>(st, q, nx, hasnext, nfloor, floor_live, lim32, psols, pnodes, ran);
             else
-                search_phase<T, false>(st, q, floor_live, lim32, psols, pnodes, ran);
+                search_phase<T, false>(st, q, nx, hasnext, nfloor, floor_live, lim32, psols, pnodes, ran);
+            // the phase cleared the bits of the lanes that started their next item
+            if (((had & ~hasnext) >> lane) & 1u) holdnext = false;
             // a lane whose piece ended borrowed below its stack; its last
             // load (its own row-0 slot) is void
             if ((int32_t)q < 0) {
@@ -1171,6 +1254,7 @@ hipError_t k2_drain(hipStream_t s, uint32_t blocks, uint32_t threads, const K2Dr
     a.err = p.err;
     a.stack_rows = p.stack_rows;
     a.refill_idle = p.refill_idle;
+    a.next_refill = p.next_refill ? p.next_refill : std::min(56u, 3u * p.refill_idle);
     a.donate = p.donate;
     a.budget = p.budget;
     a.cont_in = p.cont_in;

@@ -236,7 +236,7 @@ int solve_batch_impl(nqm_ctx* ctx, const nqm_solve* solves, uint32_t ns, const n
     if (tune_in) t = *tune_in;
     if (t.flags & ~(NQM_TUNE_NO_MIRROR | NQM_TUNE_NO_DONATE | NQM_TUNE_GRAPH | NQM_TUNE_NO_ORDER))
         return fail(ctx, NQM_EINVAL, "unknown tune flags 0x%x", t.flags);
-    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 8u;
+    const uint32_t refill = t.refill_idle ? std::min(t.refill_idle, 64u) : 12u;
     const bool mirror = !(t.flags & NQM_TUNE_NO_MIRROR);
     const bool use_graph = (t.flags & NQM_TUNE_GRAPH) && !t.budget;
 

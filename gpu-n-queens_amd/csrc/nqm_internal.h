@@ -158,6 +158,7 @@ struct K2DrainParams {
     uint32_t* err = nullptr;
     uint32_t stack_rows = 1;
     uint32_t refill_idle = 16;
+    uint32_t next_refill = 0;  // 0: 3 x refill_idle, at most 56
     uint32_t donate = 1;
     uint64_t budget = 0;
     // parked pieces: resumed from cont_in[0, *cont_in_count) (taken through

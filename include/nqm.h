@@ -229,7 +229,11 @@ typedef struct {
                               largest that keeps all but at most one of the
                               waves per CU of one-wave workgroups (64 when
                               waves_per_cu is set) */
-    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 8 */
+    uint32_t refill_idle;  /* idle lanes of a wave that trigger refill/donation; 0 = 12.
+                              A single-launch drain (budget 0) also gives each busy lane
+                              its next queue item, which it starts as soon as its piece
+                              ends; a wave refills once 3 x refill_idle lanes (at most
+                              56) have started theirs */
     uint64_t budget;       /* passes per wave before unfinished pieces are parked
                               and the drain relaunched (restart); 0 = one launch */
     uint32_t flags;        /* NQM_TUNE_* */
