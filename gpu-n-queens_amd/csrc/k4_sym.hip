@@ -103,9 +103,12 @@ __host__ __device__ inline SymQ symq_child(const SymQ& it, const SymCase& c, uin
 // expansion: one level of children for every pending item
 // ---------------------------------------------------------------------------
 // A child at its target row goes to the final queue, one still below it to
-// the next level. Outputs are reserved with one atomic per wave; the order
-// within a level does not matter (every item is an independent subtree).
-// Counts: cnt[0] final, cnt[1] next level.
+// the next level. Three launches per level, none with a same-address
+// atomic: k4_expand_count (block b, items [256 b, 256 b + 256): its
+// children of each kind and the placements), k4_expand_scan (one block: the
+// blocks' offsets, the level's sizes in cnt[0..1], the placements added to
+// nodes), k4_expand_write (block b writes its children at its offsets, in
+// item order).
 __device__ __forceinline__ uint32_t k4_incl_scan(uint32_t x, uint32_t lane) {
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -115,93 +118,151 @@ __device__ __forceinline__ uint32_t k4_incl_scan(uint32_t x, uint32_t lane) {
     return x;
 }
 
+// The children of item i of each kind (final, next).
+__device__ __forceinline__ void k4_child_counts(const SymQ& it, uint32_t R, uint32_t& nf, uint32_t& nn) {
+    const SymCase c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
+    const uint32_t tgt = symq_target(it.meta, R);
+    nf = 0;
+    nn = 0;
+    for (uint32_t f = it.f; f; f &= f - 1u) {
+        const SymQ ch = symq_child(it, c, f & (0u - f));
+        if (ch.f) {
+            if (symq_row(ch.meta) >= tgt)
+                ++nf;
+            else
+                ++nn;
+        }
+    }
+}
+
+// Block sums of 256 per-thread values (a, b) over the block; every thread
+// gets its exclusive prefix and the block totals.
+__device__ __forceinline__ void k4_block_scan2(uint32_t a, uint32_t b, uint32_t& pa, uint32_t& pb, uint32_t& ta,
+                                               uint32_t& tb, uint2* wsum) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ia = k4_incl_scan(a, lane), ib = k4_incl_scan(b, lane);
+    if (lane == 63u) wsum[wave] = make_uint2(ia, ib);
+    __syncthreads();
+    uint32_t ba = 0, bb = 0;
+    ta = 0;
+    tb = 0;
+    for (uint32_t w = 0; w < 4u; ++w) {
+        const uint2 v = wsum[w];
+        if (w < wave) {
+            ba += v.x;
+            bb += v.y;
+        }
+        ta += v.x;
+        tb += v.y;
+    }
+    pa = ba + ia - a;
+    pb = bb + ib - b;
+}
+
 __global__ void __launch_bounds__(256)
-k4_expand_count_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, uint32_t* __restrict__ cnt) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t fin = 0, nxt = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+k4_expand_count_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, uint4* __restrict__ part) {
+    __shared__ uint2 wsum[4];
+    __shared__ uint32_t wplaced[4];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t nf = 0, nn = 0, placed = 0;
+    if (i < count) {
         const SymQ it = in[i];
+        k4_child_counts(it, R, nf, nn);
+        placed = (uint32_t)__popc(it.f);
+    }
+    uint32_t pa, pb, ta, tb;
+    k4_block_scan2(nf, nn, pa, pb, ta, tb, wsum);
+    for (int o = 32; o > 0; o >>= 1) placed += __shfl_xor(placed, o, 64);
+    if (lane == 0) wplaced[wave] = placed;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = make_uint4(ta, tb, wplaced[0] + wplaced[1] + wplaced[2] + wplaced[3], 0u);
+}
+
+// One block: part[b].x / .y become block b's offsets in the final / next
+// queue (exclusive scans); cnt[0], cnt[1] = the level's final and next
+// sizes; the level's placements go to nodes.
+__global__ void __launch_bounds__(1024)
+k4_expand_scan_kernel(uint4* __restrict__ part, uint32_t nblocks, uint32_t* __restrict__ cnt,
+                      unsigned long long* __restrict__ nodes) {
+    __shared__ uint2 wsum[16];
+    __shared__ unsigned long long wpl[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    uint32_t ca = 0, cb = 0;  // block-uniform carries
+    uint64_t placed = 0;
+    for (uint32_t base = 0; base < nblocks; base += 1024u) {
+        const uint32_t i = base + tid;
+        const uint4 v = i < nblocks ? part[i] : make_uint4(0u, 0u, 0u, 0u);
+        placed += v.z;
+        const uint32_t ia = k4_incl_scan(v.x, lane), ib = k4_incl_scan(v.y, lane);
+        if (lane == 63u) wsum[wave] = make_uint2(ia, ib);
+        __syncthreads();
+        uint32_t ba = 0, bb = 0, ta = 0, tb = 0;
+        for (uint32_t w = 0; w < 16u; ++w) {
+            const uint2 s = wsum[w];
+            if (w < wave) {
+                ba += s.x;
+                bb += s.y;
+            }
+            ta += s.x;
+            tb += s.y;
+        }
+        if (i < nblocks) part[i] = make_uint4(ca + ba + ia - v.x, cb + bb + ib - v.y, v.z, 0u);
+        ca += ta;
+        cb += tb;
+        __syncthreads();
+    }
+    for (int o = 32; o > 0; o >>= 1) placed += __shfl_xor(placed, o, 64);
+    if (lane == 0) wpl[wave] = placed;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long sp = 0;
+        for (uint32_t w = 0; w < 16u; ++w) sp += wpl[w];
+        cnt[0] = ca;
+        cnt[1] = cb;
+        if (sp) atomicAdd(nodes, sp);
+    }
+}
+
+// Block b writes its children: final ones at fin_q[fin_at + offset] (the
+// final queue already holds fin_at items), next ones at next_q[offset].
+// Offsets past a capacity set err bit 1 (the level was not the counted one).
+__global__ void __launch_bounds__(256)
+k4_expand_write_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, const uint4* __restrict__ part,
+                       SymQ* __restrict__ fin_q, uint32_t fin_at, uint32_t fin_cap, SymQ* __restrict__ next_q,
+                       uint32_t next_cap, uint32_t* __restrict__ err) {
+    __shared__ uint2 wsum[4];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    SymQ it{};
+    uint32_t nf = 0, nn = 0;
+    if (i < count) {
+        it = in[i];
+        k4_child_counts(it, R, nf, nn);
+    }
+    uint32_t pa, pb, ta, tb;
+    k4_block_scan2(nf, nn, pa, pb, ta, tb, wsum);
+    if (nf | nn) {
+        const uint4 off = part[blockIdx.x];
+        uint32_t bf = fin_at + off.x + pa, bn = off.y + pb;
         const SymCase c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
         const uint32_t tgt = symq_target(it.meta, R);
         for (uint32_t f = it.f; f; f &= f - 1u) {
             const SymQ ch = symq_child(it, c, f & (0u - f));
-            if (ch.f) {
-                if (symq_row(ch.meta) >= tgt)
-                    ++fin;
+            if (!ch.f) continue;
+            if (symq_row(ch.meta) >= tgt) {
+                if (bf < fin_cap)
+                    fin_q[bf] = ch;
                 else
-                    ++nxt;
+                    atomicOr(err, 1u);
+                ++bf;
+            } else {
+                if (bn < next_cap)
+                    next_q[bn] = ch;
+                else
+                    atomicOr(err, 1u);
+                ++bn;
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        fin += __shfl_xor(fin, o, 64);
-        nxt += __shfl_xor(nxt, o, 64);
-    }
-    if (lane == 0) {
-        if (fin) atomicAdd(&cnt[0], fin);
-        if (nxt) atomicAdd(&cnt[1], nxt);
-    }
-}
-
-__global__ void __launch_bounds__(256)
-k4_expand_write_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, SymQ* __restrict__ fin_q,
-                       uint32_t fin_cap, SymQ* __restrict__ next_q, uint32_t next_cap, uint32_t* __restrict__ cnt,
-                       unsigned long long* __restrict__ nodes, uint32_t* __restrict__ err) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint64_t placed = 0;
-    // block-uniform trip count: every lane takes part in the scans
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < count; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
-        SymQ it{};
-        uint32_t nf = 0, nn = 0;
-        SymCase c{};
-        uint32_t tgt = 0;
-        if (i < count) {
-            it = in[i];
-            c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
-            tgt = symq_target(it.meta, R);
-            placed += (uint32_t)__popc(it.f);
-            for (uint32_t f = it.f; f; f &= f - 1u) {
-                const SymQ ch = symq_child(it, c, f & (0u - f));
-                if (ch.f) {
-                    if (symq_row(ch.meta) >= tgt)
-                        ++nf;
-                    else
-                        ++nn;
-                }
-            }
-        }
-        const uint32_t sf = k4_incl_scan(nf, lane), sn = k4_incl_scan(nn, lane);
-        uint32_t bf = 0, bn = 0;
-        if (lane == 63u) {
-            if (sf) bf = atomicAdd(&cnt[2], sf);
-            if (sn) bn = atomicAdd(&cnt[3], sn);
-        }
-        bf = __shfl(bf, 63, 64) + sf - nf;
-        bn = __shfl(bn, 63, 64) + sn - nn;
-        if (nf | nn) {
-            for (uint32_t f = it.f; f; f &= f - 1u) {
-                const SymQ ch = symq_child(it, c, f & (0u - f));
-                if (!ch.f) continue;
-                if (symq_row(ch.meta) >= tgt) {
-                    if (bf < fin_cap)
-                        fin_q[bf] = ch;
-                    else
-                        atomicOr(err, 1u);
-                    ++bf;
-                } else {
-                    if (bn < next_cap)
-                        next_q[bn] = ch;
-                    else
-                        atomicOr(err, 1u);
-                    ++bn;
-                }
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) placed += __shfl_xor(placed, o, 64);
-    if (lane == 0 && placed) atomicAdd(nodes, (unsigned long long)placed);
 }
 
 // ---------------------------------------------------------------------------
@@ -941,28 +1002,30 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
     // the final queue starts with the host's final items
     if (!known && (rc = grow_keep(ctx, w.fin, 0, (size_t)std::max<uint32_t>(host_fin, 1) * sizeof(SymQ))) != NQM_OK)
         return rc;
-    if (host_fin) {
+    if (host_fin)
         NQM_HIP(ctx, hipMemcpyAsync(w.fin.ptr, w.items.ptr, (size_t)host_fin * sizeof(SymQ),
                                     hipMemcpyDeviceToDevice, st));
-        NQM_HIP(ctx, hipMemcpyAsync(d_cnt + 2, &host_fin, 4, hipMemcpyHostToDevice, st));
-    }
     // levels: every pending item grows by one row; children at their target
-    // row are appended to the final queue (cursor d_cnt[2]), the others form
-    // the next level (cursor d_cnt[3])
+    // row are appended to the final queue (which holds fin_at items), the
+    // others form the next level
     std::vector<uint32_t> level_in, level_fin;
+    uint64_t fin_at = host_fin;
     const SymQ* pend = static_cast<const SymQ*>(w.items.ptr) + host_fin;
     uint32_t npend = host_pend;
     uint32_t L = 0;
     while (npend) {
-        const uint32_t grid = std::min<uint32_t>(4096u, (npend + 255u) / 256u);
+        const uint32_t blocks = (npend + 255u) / 256u;
+        if ((rc = ensure_dev(ctx, w.xpart, (size_t)blocks * sizeof(uint4))) != NQM_OK) return rc;
+        uint4* const d_xpart = static_cast<uint4*>(w.xpart.ptr);
+        hipLaunchKernelGGL(k4_expand_count_kernel, dim3(blocks), dim3(256), 0, st, pend, npend, R, d_xpart);
+        NQM_HIP(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k4_expand_scan_kernel, dim3(1), dim3(1024), 0, st, d_xpart, blocks, d_cnt, d_xnodes);
+        NQM_HIP(ctx, hipGetLastError());
         uint32_t nfin_l = 0, nnext = 0;
         if (known) {
             nfin_l = w.level_fin[L];
             nnext = L + 1 < w.level_in.size() ? w.level_in[L + 1] : 0u;
         } else {
-            NQM_HIP(ctx, hipMemsetAsync(d_cnt, 0, 8, st));
-            hipLaunchKernelGGL(k4_expand_count_kernel, dim3(grid), dim3(256), 0, st, pend, npend, R, d_cnt);
-            NQM_HIP(ctx, hipGetLastError());
             uint32_t hc[2] = {0, 0};
             NQM_HIP(ctx, hipMemcpyAsync(hc, d_cnt, 8, hipMemcpyDeviceToHost, st));
             NQM_HIP(ctx, hipStreamSynchronize(st));
@@ -976,14 +1039,14 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
                 return rc;
             level_in.push_back(npend);
             level_fin.push_back(nfin_l);
-            nfin64 += nfin_l;
         }
         SymQ* const nxt = static_cast<SymQ*>(w.pend[(L + 1) & 1].ptr);
-        NQM_HIP(ctx, hipMemsetAsync(d_cnt + 3, 0, 4, st));
-        hipLaunchKernelGGL(k4_expand_write_kernel, dim3(grid), dim3(256), 0, st, pend, npend, R,
-                           static_cast<SymQ*>(w.fin.ptr), (uint32_t)nfin64,
-                           nxt, nnext, d_cnt, d_xnodes, d_err);
+        hipLaunchKernelGGL(k4_expand_write_kernel, dim3(blocks), dim3(256), 0, st, pend, npend, R, d_xpart,
+                           static_cast<SymQ*>(w.fin.ptr), (uint32_t)fin_at, (uint32_t)(fin_at + nfin_l), nxt, nnext,
+                           d_err);
         NQM_HIP(ctx, hipGetLastError());
+        fin_at += nfin_l;
+        if (!known) nfin64 += nfin_l;
         pend = nxt;
         npend = nnext;
         ++L;
@@ -1125,7 +1188,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         stats->kernel_ms = all_ms;
         stats->search_ms = search_ms;
         stats->wall_ms = now_ms() - t0;
-        stats->launches = (known ? 1u : 2u) * L + 3u;
+        stats->launches = 3u * L + 3u;
         stats->blocks = blocks;
         stats->rounds = 1;
         stats->lanes = (uint64_t)blocks * kK4Threads;

@@ -219,7 +219,7 @@ void nqm_destroy(nqm_ctx* ctx) {
                            &w.hist, &w.stat, &w.goal})
         free_dev(*d);
     nqm::SymWork& sw = ctx->sym;
-    for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.recs, &sw.rec_counts,
+    for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.xpart, &sw.recs, &sw.rec_counts,
                            &sw.boards, &sw.mult, &sw.wave})
         free_dev(*d);
     nqm::FsWork& fw = ctx->fs;
