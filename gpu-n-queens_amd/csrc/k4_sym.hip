@@ -283,6 +283,7 @@ struct K4Args {
     uint32_t* head;       // nseg heads, kHeadStride u32 apart
     uint32_t n, last, mask, side;
     uint32_t refill_idle;
+    uint32_t next_refill;  // a phase ends once this many lanes started their next item
     uint4* recs;          // record chunks (kRecChunk records {item, ld, rd, 0} each); chunk nchunks is the overflow sink
     uint32_t* rec_counts; // records in each chunk, written when the wave closes it
     uint32_t* chunk_head; // chunks handed out
@@ -344,11 +345,28 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
 // bitop3 truth tables index (S0 S1 S2) as a 3-bit number, S0 the high bit:
 // 0x14 = (a ^ b) & ~c, 0xd4 = c ? a & b : a | b, 0xf8 = a | (b & c),
 // 0x10 = a & ~b & ~c.
-#define NQM_K4_POP(U, CX, CT)                              \
+#define NQM_K4_POP(U, CX, CA, CL, CR, CT)                  \
     "v_cmpx_eq_u32 vcc, 0, " CX "\n"                       \
     "v_subrev_co_u32 %[q], vcc, %[row], %[q]\n"            \
     "ds_read_b128 " CT ", %[q] offset:%[row]\n"            \
     "s_cbranch_vccz .Lk4_nd" #U "_%=\n"                    \
+    "s_and_b64 exec, vcc, %[hasnext]\n"                    \
+    "s_cbranch_execz .Lk4_nn" #U "_%=\n"                   \
+    "s_waitcnt lgkmcnt(0)\n"                               \
+    "v_mov_b32 " CX ", v100\n"                             \
+    "v_mov_b32 " CA ", v101\n"                             \
+    "v_mov_b32 " CL ", v102\n"                             \
+    "v_mov_b32 " CR ", v103\n"                             \
+    "v_mov_b32 %[lm], v104\n"                              \
+    "v_mov_b32 %[bitb1], v105\n"                           \
+    "v_mov_b32 %[endb], v106\n"                            \
+    "v_mov_b32 %[amask], v107\n"                           \
+    "v_mov_b32 %[sidel], v108\n"                           \
+    "v_mov_b32 v96, v109\n"                                \
+    "v_add_u32 %[q], %[row], %[q]\n"                       \
+    "s_andn2_b64 %[hasnext], %[hasnext], exec\n"           \
+    "s_andn2_b64 vcc, vcc, exec\n"                         \
+    ".Lk4_nn" #U "_%=:\n"                                  \
     "s_andn2_b64 %[alive], %[alive], vcc\n"                \
     "s_bcnt1_i32_b64 %[t], vcc\n"                          \
     "s_add_u32 %[t2], %[ran], " #U "\n"                    \
@@ -400,24 +418,37 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "s_add_u32 %[cnt], %[cnt], %[t]\n"                     \
     "s_mov_b64 exec, %[alive]\n"                           \
     ".Lk4_ne" #U "_%=:\n"
-#define NQM_K4_PASS_A(U) NQM_K4_POP(U, "v88", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95")
-#define NQM_K4_PASS_B(U) NQM_K4_POP(U, "v92", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91")
+#define NQM_K4_PASS_A(U) NQM_K4_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95")
+#define NQM_K4_PASS_B(U) NQM_K4_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91")
 #define NQM_K4_PASS8 NQM_K4_PASS_A(0) NQM_K4_PASS_B(1) NQM_K4_PASS_A(2) NQM_K4_PASS_B(3) \
                      NQM_K4_PASS_A(4) NQM_K4_PASS_B(5) NQM_K4_PASS_A(6) NQM_K4_PASS_B(7)
 
 typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
 
-// Runs passes until at most floor_live lanes hold a piece, lim32 passes ran
-// or the chunk holds kRecLimit records (checked every 8 passes). Returns the
-// non-tie full boards (psols) and the placements (pnodes) of the wave.
-__device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, uint32_t bitb1, uint32_t endb,
-                                         uint32_t amask, uint32_t sidel, uint32_t item, uint32_t& cnt,
+typedef uint32_t k4_v8 __attribute__((ext_vector_type(8)));
+
+// The rule constants of a lane's piece (see take() in k4_drain_kernel).
+struct K4Rules {
+    uint32_t lm, bitb1, endb, amask, sidel, item;
+};
+
+// Runs passes until at most floor_live lanes hold a piece, at most nfloor
+// lanes (signed: -1 never) still hold an unstarted next item, lim32 passes
+// ran or the chunk holds kRecLimit records (checked every 8 passes). A lane
+// in hasnext whose piece ends starts its next item (nst, nr) in the same
+// pass, and its bit leaves hasnext. Returns the non-tie full boards (psols)
+// and the placements (pnodes) of the wave.
+__device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, const uint4& nst, const K4Rules& nr,
+                                         uint64_t& hasnext, int32_t nfloor, uint32_t& cnt,
                                          uint64_t recbase, uint32_t sn,
                                          uint32_t snm1, uint32_t sside, uint32_t floor_live, uint32_t lim32,
                                          uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     k4_v4 s4 = {st.x, st.y, st.z, st.w};
     k4_v4 s5;
-    k4_v4 rec = {item, 0u, 0u, 0u};
+    k4_v4 rec = {rl.item, 0u, 0u, 0u};
+    const k4_v4 n4 = {nst.x, nst.y, nst.z, nst.w};
+    const k4_v8 n8 = {nr.lm, nr.bitb1, nr.endb, nr.amask, nr.sidel, nr.item, 0u, 0u};
+    uint32_t lm = rl.lm, bitb1 = rl.bitb1, endb = rl.endb, amask = rl.amask, sidel = rl.sidel;
     uint64_t live, alive, sm, se;
     uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
     uint32_t sols = 0, nodes = 0;
@@ -433,6 +464,9 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, ui
         "s_bcnt1_i32_b64 %[t], exec\n"
         "s_cmp_le_u32 %[t], %[floor]\n"
         "s_cbranch_scc1 .Lk4_end_%=\n"
+        "s_bcnt1_i32_b64 %[t], %[hasnext]\n"
+        "s_cmp_le_i32 %[t], %[nfloor]\n"
+        "s_cbranch_scc1 .Lk4_end_%=\n"
         "s_cmp_ge_u32 %[cnt], %[clim]\n"
         "s_cbranch_scc1 .Lk4_end_%=\n"
         "s_cmp_lt_u32 %[ran], %[lim]\n"
@@ -446,14 +480,14 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, uint32_t lm, ui
         : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:99]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng),
           [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live),
           [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
-          [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c)
-        : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [lm] "v"(lm),
-          [bitb1] "v"(bitb1), [endb] "v"(endb), [amask] "v"(amask), [sidel] "v"(sidel), [sn] "s"(sn),
-          [snm1] "s"(snm1), [sside] "s"(sside),
-          [recbase] "s"(recbase)
+          [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c), [lm] "+v"(lm), [bitb1] "+v"(bitb1),
+          [endb] "+v"(endb), [amask] "+v"(amask), [sidel] "+v"(sidel), [hasnext] "+s"(hasnext)
+        : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [sn] "s"(sn),
+          [snm1] "s"(snm1), [sside] "s"(sside), [recbase] "s"(recbase), [nfloor] "s"(nfloor),
+          "{v[100:103]}"(n4), "{v[104:111]}"(n8)
         : "vcc", "scc", "memory");
     (void)s5;
-    (void)rec;
+    rl = K4Rules{lm, bitb1, endb, amask, sidel, rec.x};
     st = make_uint4(s4.x, s4.y, s4.z, s4.w);
     cnt = c;
     psols = sols;
@@ -475,8 +509,14 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
 
     uint4 st = make_uint4(0u, 0u, 0u, 0u);
     uint32_t q = base;
-    uint32_t lm = 0, bitb1 = 0, endb = 0, amask = 0, sidel = 0, item = 0;
+    K4Rules rl{};  // the rule constants of the lane's piece
     auto holds = [&]() { return st.x != 0u || q != base; };
+    // Next item: a lane that holds a piece also holds the queue item it
+    // starts when the piece ends (inside the search phase: hasnext).
+    uint4 nst = make_uint4(0u, 0u, 0u, 0u);
+    K4Rules nrl{};
+    bool holdnext = false;
+    uint64_t hasnext = 0;  // wave-uniform
 
     // the queue: nseg segments with a head each (as in K2's drain)
     const uint32_t nseg = a.nseg;
@@ -507,72 +547,86 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         }
     };
 
-    auto take = [&](uint32_t idx) {
+    // the state and rule constants of queue item idx
+    auto load_piece = [&](uint32_t idx, uint4& ps, K4Rules& pr) {
         const SymQ it = a.items[idx];
         const uint32_t r = symq_row(it.meta), b1 = symq_b1(it.meta);
         uint32_t ld = it.ld, rd = it.rd, avail = ~it.cols & a.mask;
-        lm = 0;
-        bitb1 = 0;
-        endb = 0;
-        amask = 0;
-        sidel = 0;
+        pr = K4Rules{0u, 0u, 0u, 0u, 0u, idx};
         if (symq_corner(it.meta)) {
             if (b1 > r) {  // rows below b1 avoid column 1: out of avail until the child at row b1
                 avail &= ~2u;
                 ld |= 1u << (31u - (b1 - r));
-                amask = 2u;
+                pr.amask = 2u;
             }
         } else {
-            sidel = a.side;
+            pr.sidel = a.side;
             const SymCase c = sym_case(a.n, b1, 0u);
             if (c.b2 > r) ld |= 1u << (31u - (c.b2 - r));  // reaches bit 31 at child row b2
             rd |= 1u << (a.n + (a.last - r));              // reaches bit n at the last row
-            lm = c.lastmask;
-            bitb1 = 1u << b1;
-            endb = (b1 < r && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
+            pr.lm = c.lastmask;
+            pr.bitb1 = 1u << b1;
+            pr.endb = (b1 < r && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
         }
-        st = make_uint4(it.f, avail, ld, rd);
-        q = base;
-        item = idx;
+        ps = make_uint4(it.f, avail, ld, rd);
+    };
+    // the queue items for the lanes of `want` (wave-collective): the held
+    // range first, then a new grab; returns the lane's item or kNoItem
+    constexpr uint32_t kNoItem = 0xffffffffu;
+    auto take_queue = [&](uint64_t want) -> uint32_t {
+        const uint32_t nw = __popcll(want);
+        const uint32_t take_local = min(nw, lq_end - lq);
+        uint32_t g = 0, got = 0;
+        if (take_local < nw && queue_open) {
+            const uint32_t need = nw - take_local;
+            uint32_t wn = rem / (2u * seg_waves);
+            wn = wn < need ? need : (wn > kK4Grab ? kK4Grab : wn);
+            const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
+            if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, wn);
+            g = __shfl(g, 0, 64);
+            got = g >= len ? 0u : min(wn, len - g);
+            rem = g >= len ? 0u : len - g - got;
+            g += lo;
+            if (got < wn) {
+                if (++seg_done == nseg) queue_open = false;
+                seg = seg + 1u == nseg ? 0u : seg + 1u;
+                rem = seg_lo(seg + 1) - seg_lo(seg);
+            }
+        }
+        const uint32_t r = k4_rank(want);
+        uint32_t idx = kNoItem;
+        if ((want >> lane) & 1u) {
+            if (r < take_local)
+                idx = lq + r;
+            else if (r - take_local < got)
+                idx = g + (r - take_local);
+        }
+        lq += take_local;
+        if (got) {
+            lq = g + min(got, nw - take_local);
+            lq_end = g + got;
+        }
+        return idx;
     };
 
     uint64_t leaves = 0, exec_nodes = 0;
     for (;;) {
+        // a lane whose piece ended outside the phase (its last child was
+        // dead and its stack empty when the phase stopped) starts its next
+        // item here
+        if (!holds() && holdnext) {
+            st = nst;
+            rl = nrl;
+            q = base;
+            holdnext = false;
+        }
         uint64_t idle = __ballot(!holds());
         if (idle) {
-            const uint32_t nidle = __popcll(idle);
             if (queue_open || lq < lq_end) {
-                const uint32_t take_local = min(nidle, lq_end - lq);
-                uint32_t g = 0, got = 0;
-                if (take_local < nidle && queue_open) {
-                    const uint32_t need = nidle - take_local;
-                    uint32_t want = rem / (2u * seg_waves);
-                    want = want < need ? need : (want > kK4Grab ? kK4Grab : want);
-                    const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1) - lo;
-                    if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, want);
-                    g = __shfl(g, 0, 64);
-                    got = g >= len ? 0u : min(want, len - g);
-                    rem = g >= len ? 0u : len - g - got;
-                    g += lo;
-                    if (got < want) {
-                        if (++seg_done == nseg) queue_open = false;
-                        seg = seg + 1u == nseg ? 0u : seg + 1u;
-                        rem = seg_lo(seg + 1) - seg_lo(seg);
-                    }
-                }
-                const uint32_t r = k4_rank(idle);
-                uint32_t idx = 0xffffffffu;
-                if (!holds()) {
-                    if (r < take_local)
-                        idx = lq + r;
-                    else if (r - take_local < got)
-                        idx = g + (r - take_local);
-                }
-                if (idx != 0xffffffffu) take(idx);
-                lq += take_local;
-                if (got) {
-                    lq = g + min(got, nidle - take_local);
-                    lq_end = g + got;
+                const uint32_t idx = take_queue(idle);
+                if (idx != kNoItem) {
+                    load_piece(idx, st, rl);
+                    q = base;
                 }
                 idle = __ballot(!holds());
             }
@@ -592,17 +646,12 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                     const uint32_t ir = k4_rank(idle);
                     const bool recv = !holds() && ir < k;
                     const uint32_t dl = recv ? wscratch[ir] : lane;
-                    const uint32_t d_lm = __shfl(lm, dl, 64), d_bitb1 = __shfl(bitb1, dl, 64);
-                    const uint32_t d_endb = __shfl(endb, dl, 64), d_item = __shfl(item, dl, 64);
-                    const uint32_t d_amask = __shfl(amask, dl, 64), d_sidel = __shfl(sidel, dl, 64);
+                    const K4Rules d{(uint32_t)__shfl(rl.lm, dl, 64),    (uint32_t)__shfl(rl.bitb1, dl, 64),
+                                    (uint32_t)__shfl(rl.endb, dl, 64),  (uint32_t)__shfl(rl.amask, dl, 64),
+                                    (uint32_t)__shfl(rl.sidel, dl, 64), (uint32_t)__shfl(rl.item, dl, 64)};
                     if (recv) {
                         st = k4_slot_load(base - 16u * lane + 16u * dl + row);  // the donor's slot 0
-                        lm = d_lm;
-                        bitb1 = d_bitb1;
-                        endb = d_endb;
-                        amask = d_amask;
-                        sidel = d_sidel;
-                        item = d_item;
+                        rl = d;
                         q = base;
                     }
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -616,6 +665,19 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                 }
             }
         }
+        // lanes with a piece and no next item take one
+        if (queue_open || lq < lq_end) {
+            const uint64_t want = __ballot(holds() && !holdnext);
+            if (want) {
+                const uint32_t idx = take_queue(want);
+                if (idx != kNoItem) {
+                    load_piece(idx, nst, nrl);
+                    holdnext = true;
+                }
+            }
+        }
+        hasnext = __ballot(holdnext);
+
         const uint64_t live_mask = __ballot(holds());
         const bool have_items = queue_open || lq < lq_end;
         if (live_mask == 0 && !have_items) break;
@@ -631,6 +693,10 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         }
         floor_live = __builtin_amdgcn_readfirstlane(floor_live);
         limit = __builtin_amdgcn_readfirstlane(limit);
+        // the phase also ends once next_refill lanes have started their next
+        // item (while the queue can give them another)
+        const int32_t nfloor = __builtin_amdgcn_readfirstlane(
+            have_items ? (int32_t)__popcll(hasnext) - (int32_t)a.next_refill : -1);
         if (chunk == kNoChunk) grab_chunk();
         const uint64_t rb = (uint64_t)(size_t)(a.recs + (size_t)chunk * kRecChunk);
         // (readfirstlane returns int: widen through uint32_t, never by sign)
@@ -638,8 +704,11 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                                  (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)rb);
         uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
         if (holds()) {
-            k4_phase(st, q, lm, bitb1, endb, amask, sidel, item, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit, psols,
-                     pnodes, ran);
+            const uint64_t had = hasnext;
+            k4_phase(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit,
+                     psols, pnodes, ran);
+            // the phase cleared the bits of the lanes that started their next item
+            if (((had & ~hasnext) >> lane) & 1u) holdnext = false;
             if ((int32_t)q < 0) {  // the piece ended: its last load is void
                 q = base;
                 st.x = 0;
@@ -1092,7 +1161,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         a.last = last;
         a.mask = (1u << n) - 1u;
         a.side = (1u << last) | 1u;
-        a.refill_idle = 8u;
+        a.refill_idle = 12u;
+        a.next_refill = 36u;
         a.recs = static_cast<uint4*>(w.recs.ptr);
         a.rec_counts = static_cast<uint32_t*>(w.rec_counts.ptr);
         a.chunk_head = d_chunk_head;
