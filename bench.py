@@ -42,6 +42,7 @@ Other workloads (one JSON line each, not the headline metric):
   --workload sym      symmetry-reduced counts (classes under the board's 8
                       symmetries and their total) of N=16..18 by both kernels:
                       the persistent drain (K4) and one thread per item (K3)
+  --workload fullsweep  full DoubleSweep counts (K5) of N=16..18
 """
 import argparse
 import json
@@ -75,7 +76,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--requested-states", type=int, default=80_000_000)
-    ap.add_argument("--workload", default="n18", choices=["n18", "batch", "n19", "serving", "sym"])
+    ap.add_argument("--workload", default="n18", choices=["n18", "batch", "n19", "serving", "sym", "fullsweep"])
     ap.add_argument("--requests", type=int, default=2000, help="serving: stream length")
     ap.add_argument("--load", type=float, default=0.7, help="serving: target utilisation")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -345,6 +346,16 @@ def workload_sym(ctx, nqm, args):
             "sizes": {str(n): r for n, r in rows.items()}}
 
 
+def workload_fullsweep(ctx, nqm, args):
+    rows = {n: fullsweep_leg(ctx, n, reps=max(1, args.steps)) for n in (16, 17, 18)}
+    d = rows[N]
+    return {"metric": "full DoubleSweep exact count at N=18: wall ms", "value": d["ms"], "unit": "ms",
+            "n_gpus": 1, "higher_is_better": False, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": "N=16..18 empty boards, full DoubleSweep (row and column propagation, "
+                                   "centre-out branching); totals against test.cu:18"},
+            "sizes": {str(n): r for n, r in rows.items()}}
+
+
 def workload_serving(ctx, nqm, args):
     import numpy as np
     from nqm import serving, workloads
@@ -530,7 +541,7 @@ def main():
 
     if args.workload != "n18":
         fn = {"batch": workload_batch, "n19": workload_n19, "serving": workload_serving,
-              "sym": workload_sym}[args.workload]
+              "sym": workload_sym, "fullsweep": workload_fullsweep}[args.workload]
         line = fn(ctx, nqm, args)
         line["device"] = info
         ctx.close()

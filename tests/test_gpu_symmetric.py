@@ -54,6 +54,12 @@ def test_drain_n17_n18(ctx, known_totals):
     assert (u2, t2, st2.nodes) == (u, t, st.nodes)
 
 
+def test_drain_n19_the_largest_it_takes(ctx, known_totals):
+    unique = {int(k): v for k, v in golden_json("unique_totals.json")["unique"].items()}
+    u, t, st = ctx.count_symmetric(19, nqm.SYM_DRAIN)
+    assert u == unique[19] and t == known_totals[19], (u, t)
+
+
 def test_drain_rejects_sizes_outside_its_range(ctx):
     for n in (9, 20, 21, 22):
         rc = ctx.lib.nqm_count_symmetric_path(ctx.handle, n, nqm.SYM_DRAIN, None, None, None)
