@@ -833,6 +833,7 @@ extern "C" int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsol
     bool have_ref = false, have_best = false;
     double best_ms = 0;
     uint32_t nres = 0;
+    int last_rc = NQM_OK;
     for (uint32_t d : depths)
         for (uint32_t w : waves)
             for (uint32_t r : rows) {
@@ -842,9 +843,18 @@ extern "C" int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsol
                 t.stack_rows = r;
                 t.flags = space->base_flags & ~(uint32_t)NQM_TUNE_GRAPH;
                 std::vector<double> ms;
-                for (uint32_t k = 0; k < reps; ++k) {
+                bool feasible = true;
+                for (uint32_t k = 0; k < reps && feasible; ++k) {
                     nqm_stats st{};
                     int rc = nqm::solve_batch_impl(ctx, solves, nsolves, &t, got.data(), nullptr, &st);
+                    if (rc == NQM_ERANGE || rc == NQM_ENOMEM) {
+                        // the queue this combination asks for does not fit
+                        // (e.g. a stack-row cap that forces a deep expansion):
+                        // it is reported, not tried
+                        feasible = false;
+                        last_rc = rc;
+                        break;
+                    }
                     if (rc != NQM_OK) return rc;
                     if (!have_ref) {
                         ref = got;
@@ -856,7 +866,7 @@ extern "C" int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsol
                     ms.push_back(st.kernel_ms);
                 }
                 std::sort(ms.begin(), ms.end());
-                const double med = ms[ms.size() / 2];
+                const double med = feasible ? ms[ms.size() / 2] : -1.0;
                 if (results && nres < max_results) {
                     results[nres].depth = d;
                     results[nres].waves_per_cu = w;
@@ -865,12 +875,13 @@ extern "C" int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsol
                     results[nres].ms = med;
                 }
                 ++nres;
-                if (!have_best || med < best_ms) {
+                if (feasible && (!have_best || med < best_ms)) {
                     best_ms = med;
                     *best = t;
                     have_best = true;
                 }
             }
     if (nresults) *nresults = std::min(nres, max_results);
+    if (!have_best) return last_rc != NQM_OK ? last_rc : nqm::fail(ctx, NQM_EINVAL, "no tuning to try");
     return NQM_OK;
 }

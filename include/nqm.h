@@ -264,9 +264,12 @@ int nqm_solve_batch(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
  * expansion depths, waves per CU and LDS stack-row caps (zero-terminated
  * lists; an empty list means "automatic only"), `repeats` times each, and
  * returns the fastest tune (median kernel time). Every combination must give
- * the same counts, or the call fails with NQM_ERANGE. results (optional,
- * capacity max_results) receive one row per combination:
- * depth, waves_per_cu, stack_rows, median ms. */
+ * the same counts, or the call fails with NQM_ERANGE. A combination whose
+ * queue does not fit (a stack-row cap or depth that asks for more than 2^31
+ * items or more memory than the device has) is not run: its row reads -1 ms.
+ * The call fails only if no combination fits. results (optional, capacity
+ * max_results) receive one row per combination: depth, waves_per_cu,
+ * stack_rows, median ms. */
 typedef struct {
     uint32_t depths[8];
     uint32_t waves_per_cu[8];
