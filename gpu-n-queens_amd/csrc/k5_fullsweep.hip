@@ -235,8 +235,9 @@ __host__ __device__ __forceinline__ uint32_t fs_align(uint32_t hi, uint32_t lo, 
 }
 
 // One lane of the drain: the state of its search, advanced one step at a time
-// so that every lane of a wave does the same work per step (one line
-// evaluated, at most one pass change, branch or backtrack).
+// so that every lane of a wave does the same work per step (up to
+// kFsLinesPerStep lines evaluated, at most one pass change, branch or
+// backtrack).
 struct FsLane {
     uint32_t P, C, d1lo, d1hi, d2lo, d2hi;
     uint32_t o;        // lines of the current pass not yet evaluated
@@ -276,29 +277,35 @@ __host__ __device__ __forceinline__ void fs_lane_start(FsLane& L, const FsItem& 
 // stk[j * stride] (D1 lo, D1 hi | P << 9, D2 lo, D2 hi | C << 9) and its
 // untried columns at stkf[j * stride]. Returns false when the stack would
 // overflow (the lane stops; the count is void).
+#ifndef NQM_FS_LINES
+#define NQM_FS_LINES 8
+#endif
+constexpr uint32_t kFsLinesPerStep = NQM_FS_LINES;
+
 __host__ __device__ __forceinline__ bool fs_lane_step(FsLane& L, uint32_t n, uint32_t M, uint32_t stack, uint4* stk,
                                                       uint32_t* stkf, uint32_t stride) {
+    // up to kFsLinesPerStep lines of the current pass, in order, stopping at
+    // a dead line or the pass's end: the pass change, the branch and the pop
+    // below then run once per several line evaluations, not once per line
     bool dead = false;
-    if (L.o) {
-        const uint32_t line = fs_ctz(L.o);
-        L.o &= L.o - 1u;
-        const uint32_t a = fs_align(L.d1hi, L.d1lo, line);
-        uint32_t x, b;
-        if (L.col) {
-            x = L.P;
-            b = __builtin_bitreverse32(fs_align(L.d2hi, L.d2lo, line)) >> (32u - n);
-        } else {
-            x = L.C;
-            b = fs_align(L.d2hi, L.d2lo, n - 1u - line);
-        }
-        const uint32_t F = ~(x | a | b) & M;
-        if (F == 0u) {
-            dead = true;
-        } else if ((F & (F - 1u)) == 0u) {
-            const uint32_t q = fs_ctz(F);
-            fs_lane_place(L, L.col ? q : line, L.col ? line : q, n);
-            ++L.nodes;
-            L.changed = 1u;
+#pragma unroll
+    for (uint32_t i = 0; i < kFsLinesPerStep; ++i) {
+        if (L.o && !dead) {
+            const uint32_t line = fs_ctz(L.o);
+            L.o &= L.o - 1u;
+            const uint32_t a = fs_align(L.d1hi, L.d1lo, line);
+            const uint32_t x = L.col ? L.P : L.C;
+            const uint32_t b = L.col ? __builtin_bitreverse32(fs_align(L.d2hi, L.d2lo, line)) >> (32u - n)
+                                     : fs_align(L.d2hi, L.d2lo, n - 1u - line);
+            const uint32_t F = ~(x | a | b) & M;
+            if (F == 0u) {
+                dead = true;
+            } else if ((F & (F - 1u)) == 0u) {
+                const uint32_t q = fs_ctz(F);
+                fs_lane_place(L, L.col ? q : line, L.col ? line : q, n);
+                ++L.nodes;
+                L.changed = 1u;
+            }
         }
     }
     bool child = false, popped = false;
@@ -419,7 +426,15 @@ extern "C" int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_
     const double t0 = now_ms();
     NQM_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    const uint32_t target = target_states ? target_states : (1u << 22);
+    // Automatic (target_states 0): expand until a level holds 2^19 states
+    // and a lane's stack needs at most kFsRowsGoal rows, while levels stay
+    // under 2^26 states. Measured on the drain at N=15..18: more stack rows
+    // cost resident waves (N=18: 79.7 ms at 10 rows, 69.4 at 9), more levels
+    // past that cost a state's start for too few placements (N=16: 2.3 ms
+    // at 8 rows, 5.3 at 7).
+    constexpr uint32_t kFsRowsGoal = 9, kFsMinStates = 1u << 19, kFsMaxStates = 1u << 26;
+    const bool automatic = target_states == 0;
+    const uint32_t target = automatic ? kFsMinStates : target_states;
     FsWork& W = ctx->fs;
 
     // the root's children: row m, left-half columns weigh 2, the middle of odd n 1
@@ -448,7 +463,9 @@ extern "C" int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_
     uint32_t cur = 0, count = (uint32_t)roots.size(), levels = 0;
     // a state at level L holds at least L + 1 queens; a lane's stack holds
     // one branch point per open row but the last
-    while (count > 0 && count < target && levels < kFsMaxLevels) {
+    auto stack_rows_after = [&](uint32_t lv) { return lv + 2u < n ? n - lv - 2u : 1u; };  // placed_min = lv + 1
+    while (count > 0 && levels < kFsMaxLevels &&
+           (count < target || (automatic && stack_rows_after(levels) > kFsRowsGoal && count < kFsMaxStates))) {
         const uint64_t cap64 = (uint64_t)count * n;
         if (cap64 > 0xffffffffull) return fail(ctx, NQM_ERANGE, "level of %u states too large", count);
         const uint32_t cap = (uint32_t)cap64;

@@ -321,9 +321,10 @@ int nqm_count_symmetric_path(nqm_ctx* ctx, uint32_t n, uint32_t path, uint64_t* 
  * search_ms = the drain alone. */
 int nqm_count_fullsweep(nqm_ctx* ctx, uint32_t n, uint64_t* total, nqm_stats* stats);
 /* The same with the number of states the device expansion stops at (the
- * first level holding at least target_states; 0 = 2^22: at n = 17, 18 the
- * drain's waves then end within 5% of each other). Any target gives the same
- * total and stats->nodes. */
+ * first level holding at least target_states). 0 = automatic, what
+ * nqm_count_fullsweep does: the first level that holds 2^19 states and
+ * leaves each drain lane at most 9 stack rows (levels stay under 2^26
+ * states). Any target gives the same total and stats->nodes. */
 int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_states, uint64_t* total,
                            nqm_stats* stats);
 
