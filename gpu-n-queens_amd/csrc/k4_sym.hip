@@ -764,6 +764,10 @@ struct K4TieArgs {
 };
 
 constexpr uint32_t kTieThreads = 256;
+#ifndef NQM_TIE_STEPS
+#define NQM_TIE_STEPS 4
+#endif
+constexpr uint32_t kTieSteps = NQM_TIE_STEPS;  // search steps per lane between two refill checks
 
 // Persistent lanes: a lane takes its next record slot when it is done with
 // its record (the slots of lane g of the grid are g, g + G, g + 2 G, ...:
@@ -831,7 +835,11 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
         }
         if (exhausted && __ballot(act) == 0) break;
         bool found = false;
-        if (act) {
+        // up to kTieSteps steps of the lane's search before the wave's
+        // refill and board bookkeeping run again; a board found ends them
+#pragma unroll
+        for (uint32_t step = 0; step < kTieSteps; ++step) {
+            if (!act || found) continue;
             if (f == 0u) {
                 if (y == r0) {
                     a.mult[slot] = m;
