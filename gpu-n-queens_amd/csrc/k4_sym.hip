@@ -45,6 +45,7 @@
 
 #include "nqm_internal.h"
 #include "sym_rules.h"
+#include "k4_ties.h"
 
 namespace nqm {
 namespace {
@@ -773,34 +774,24 @@ constexpr uint32_t kTieSteps = NQM_TIE_STEPS;  // search steps per lane between 
 // its record (the slots of lane g of the grid are g, g + G, g + 2 G, ...:
 // no counter that every wave of the chip would update; a wave refills once
 // 16 of its lanes idle, so the loads of a refill are shared), and lanes of
-// a wave are not held by the longest search of the wave. Rows above the last keep
-// only the columns whose diagonal bits are in the record's ld and rd (a
-// column whose rd bit falls below bit 0 by the last row is not limited by
-// rd), which leaves few candidates per row. Going back up
-// a row rebuilds that row's cols (cols ^ b) and ld (ld >> 1 without b: the
-// bits this loses end above bit 31 by the last row); rd and the candidates
-// left are kept per row in LDS ([row][thread]). A board found is written
-// out {item, slot, path} for k4_check into the wave's board chunk (one
-// atomic per chunk); the slot's count of boards goes to mult[slot] when its
-// search ends.
+// a wave are not held by the longest search of the wave. The search is
+// TieSearch (k4_ties.h): bottom-up from the last row but one, each row's
+// columns limited by the record's ld and rd; its open rows ({candidates
+// left, up-left diagonal}) are kept in LDS ([row][thread]). A board found is
+// written out {item, slot, path} for k4_check into the wave's board chunk
+// (one atomic per chunk); the slot's count of boards goes to mult[slot]
+// when its search ends.
 __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
     __shared__ uint2 stk[kK4MaxRows * kTieThreads];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t n = a.n, last = n - 1u, mask = (1u << n) - 1u;
+    const uint32_t n = a.n, mask = (1u << n) - 1u;
     const uint32_t gsize = gridDim.x * kTieThreads;
     uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next slot
     bool act = false;
-    uint32_t slot = 0, item = 0, r0 = 0, y = 0, l = 0, d = 0, r = 0, f = 0, L = 0, Rd = 0, m = 0;
-    uint32_t endb = 0, bitb1 = 0;
-    uint64_t path = 0;
-    SymCase c = sym_case(n, 1u, 0u);
-    // a queen on column c of row yy < last sets ld bit c + (last - yy) and,
-    // when c >= last - yy, rd bit c - (last - yy) of the last row
-    auto keep = [&](uint32_t yy) {
-        if (yy >= last) return mask;
-        const uint32_t k = last - yy;
-        return (L >> k) & ((Rd << k) | ((1u << k) - 1u)) & mask;
-    };
+    uint32_t slot = 0, item = 0, m = 0;
+    TieSearch ts{};
+    auto slot_of = [&](uint32_t i) -> uint2& { return stk[i * kTieThreads + tid]; };
+    uint64_t path = 0;  // the board found
     constexpr uint32_t kNoChunk = 0xffffffffu;
     uint32_t bchunk = kNoChunk, bcnt = 0;  // wave-uniform
     for (;;) {
@@ -816,19 +807,9 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
                 const SymQ it = a.items[rc.x];
                 slot = s;
                 item = rc.x;
-                L = rc.y;
-                Rd = rc.z & mask;  // the drain's rd carries a marker above bit n-1
-                r0 = symq_row(it.meta);
-                const uint32_t b1 = symq_b1(it.meta);
-                c = sym_case(n, b1, 0u);
-                endb = (b1 < r0 && !symq_top_b1(it.meta)) ? c.endbit : 0xffffffffu;
-                bitb1 = 1u << b1;
-                y = r0;
-                l = it.ld;
-                d = it.cols;
-                r = it.rd;
-                f = it.f & keep(y);
-                path = 0;
+                // the drain's rd carries a marker above bit n-1
+                ts.start(n, rc.y, rc.z & mask, symq_row(it.meta), symq_b1(it.meta), symq_top_b1(it.meta) != 0u,
+                         it.cols, it.ld, it.rd, it.f);
                 m = 0;
                 act = true;
             }
@@ -840,35 +821,15 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
 #pragma unroll
         for (uint32_t step = 0; step < kTieSteps; ++step) {
             if (!act || found) continue;
-            if (f == 0u) {
-                if (y == r0) {
-                    a.mult[slot] = m;
-                    act = false;
-                } else {
-                    --y;
-                    const uint32_t b = 1u << (uint32_t)((path >> (5u * (y - r0))) & 31u);
-                    d ^= b;
-                    l = (l >> 1) & ~b;
-                    const uint2 e = stk[(y - r0) * kTieThreads + tid];
-                    f = e.x;
-                    r = e.y;
-                }
-            } else {
-                const uint32_t bit = f & (0u - f);
-                f ^= bit;
-                const uint32_t sh = 5u * (y - r0);
-                path = (path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
-                if (y == last) {
-                    found = l == L && r == Rd && ((bit & endb) | (l & bitb1)) != 0u;
-                    m += found ? 1u : 0u;
-                } else {
-                    stk[(y - r0) * kTieThreads + tid] = make_uint2(f, r);
-                    l = (l | bit) << 1;
-                    d |= bit;
-                    r = (r | bit) >> 1;
-                    ++y;
-                    f = sym_free(c, y, l, d, r) & keep(y);
-                }
+            uint64_t board = 0;
+            const int r = ts.step(slot_of, &board);
+            if (r == 1) {
+                a.mult[slot] = m;
+                act = false;
+            } else if (r == 2) {
+                found = true;
+                ++m;
+                path = board;
             }
         }
         const uint64_t fm = __ballot(found);
