@@ -31,11 +31,13 @@
 //    {item, ld and rd of the last row} into its wave's record chunk. Every other
 //    full board is its class's only representative and counts 8.
 //
-//  * k4_ties: one thread per record searches the item again, keeping only
-//    the columns whose diagonal bits are in the recorded ld and rd, and runs the
-//    rotation check (sym_check) on the boards it finds. Several boards of one
-//    item can share an ld, and each of them wrote the same record: a hash
-//    set keeps the first copy, which counts them all.
+//  * ties: k4_rec_compact makes the records one dense list; in
+//    k4_research one lane per record searches the item again bottom-up,
+//    keeping only the columns whose diagonal bits are in the recorded ld and
+//    rd (TieSearch, k4_ties.h), and k4_check runs the rotation check
+//    (sym_check_packed) on the boards it finds. Several boards of one item
+//    can share an ld and rd, and each of them wrote the same record: each copy
+//    counts its m boards with weight 1/m (scaled by lcm(1..16)).
 //
 // Unique classes = drain leaves + tie classes; total = 8 x drain leaves +
 // tie weights; both equal K3's and the oracle's, and so do the nodes.
@@ -287,6 +289,7 @@ struct K4Args {
     uint32_t next_refill;  // a phase ends once this many lanes started their next item
     uint4* recs;          // record chunks (kRecChunk records {item, ld, rd, 0} each); chunk nchunks is the overflow sink
     uint32_t* rec_counts; // records in each chunk, written when the wave closes it
+    uint32_t* chunk_off;  // each chunk's place in the dense record list (its share of rec_total)
     uint32_t* chunk_head; // chunks handed out
     uint32_t* rec_total;  // records written
     uint32_t nchunks;
@@ -342,7 +345,9 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
 //          rule from the open side columns of avail' & sidel (sidel = 0 for
 //          a corner item): m0 / mt = bit 0 / n-1 as 0 or ~0, blocked = side
 //          columns where both are open, all others where either is; the
-//          free set is avail' & ~ld' & ~(rd' | (blocked & t1)).
+//          free set is avail' & ~ld' & ~(rd' | (blocked & t1)). The
+//          solution and tie masks come last, and only in a pass where some
+//          lane completed a board (most passes complete none).
 // bitop3 truth tables index (S0 S1 S2) as a 3-bit number, S0 the high bit:
 // 0x14 = (a ^ b) & ~c, 0xd4 = c ? a & b : a | b, 0xf8 = a | (b & c),
 // 0x10 = a & ~b & ~c.
@@ -387,14 +392,7 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "ds_write_b128 %[q], " CT " offset:%[row]\n"           \
     "v_add_u32 %[q], %[row], %[q]\n"                       \
     "s_mov_b64 exec, %[alive]\n"                           \
-    "v_and_b32 %[e], %[b], %[endb]\n"                      \
-    "v_and_or_b32 %[e], " CL ", %[bitb1], %[e]\n"          \
-    "v_cmp_ne_u32 %[se], 0, %[e]\n"                        \
     "v_lshrrev_b32 " NR ", 1, " NR "\n"                    \
-    "s_and_b64 %[se], %[se], %[sm]\n"                      \
-    "s_andn2_b64 %[sm], %[sm], %[se]\n"                    \
-    "s_bcnt1_i32_b64 %[t], %[sm]\n"                        \
-    "s_add_u32 %[sols], %[sols], %[t]\n"                   \
     "v_ashrrev_i32 %[t1], 31, " NL "\n"                    \
     "v_bfe_i32 %[t2v], " NR ", %[sn], 1\n"                 \
     "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
@@ -406,6 +404,15 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
     "v_bitop3_b32 %[t1], " NR ", %[mt], %[t1] bitop3:0xf8\n" \
     "v_bitop3_b32 " NX ", " NA ", " NL ", %[t1] bitop3:0x10\n" \
+    "s_cmp_eq_u64 %[sm], 0\n"                              \
+    "s_cbranch_scc1 .Lk4_ne" #U "_%=\n"                    \
+    "v_and_b32 %[e], %[b], %[endb]\n"                      \
+    "v_and_or_b32 %[e], " CL ", %[bitb1], %[e]\n"          \
+    "v_cmp_ne_u32 %[se], 0, %[e]\n"                        \
+    "s_and_b64 %[se], %[se], %[sm]\n"                      \
+    "s_andn2_b64 %[sm], %[sm], %[se]\n"                    \
+    "s_bcnt1_i32_b64 %[t], %[sm]\n"                        \
+    "s_add_u32 %[sols], %[sols], %[t]\n"                   \
     "s_cmp_eq_u64 %[se], 0\n"                              \
     "s_cbranch_scc1 .Lk4_ne" #U "_%=\n"                    \
     "s_mov_b64 exec, %[se]\n"                              \
@@ -544,7 +551,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
     auto close_chunk = [&]() {
         if (lane == 0) {
             a.rec_counts[chunk] = cnt;
-            if (cnt) atomicAdd(a.rec_total, cnt);
+            a.chunk_off[chunk] = cnt ? atomicAdd(a.rec_total, cnt) : 0u;
         }
     };
 
@@ -750,63 +757,80 @@ constexpr uint64_t kMultLcm = 720720;  // lcm(1..16): every record adds lcm / m 
 
 constexpr uint32_t kBoardChunk = 1024;  // boards per chunk of a wave
 
+// Dense records: chunk ch's records go to [chunk_off[ch], + rec_counts[ch]),
+// so the search below takes no empty slot (a wave's last chunk is part full).
+__global__ void __launch_bounds__(256) k4_rec_compact_kernel(const uint4* __restrict__ recs,
+                                                            const uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ offs,
+                                                            uint4* __restrict__ dense) {
+    const uint32_t ch = blockIdx.x, cnt = counts[ch], off = offs[ch];
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256u) dense[off + i] = recs[(size_t)ch * kRecChunk + i];
+}
+
 struct K4TieArgs {
-    const uint4* recs;
-    const uint32_t* rec_counts;
-    uint32_t nslots;           // used chunks x kRecChunk
+    const uint4* recs;         // the dense records
+    uint32_t nslots;           // records
     const SymQ* items;
     uint32_t n;
     uint4* boards;             // chunks of {item, slot, path lo, path hi}; chunk nbchunks is the overflow sink
     uint32_t* board_counts;    // boards in each chunk
     uint32_t* bchunk_head;     // chunks handed out
     uint32_t nbchunks;
-    uint32_t* mult;            // per record slot: boards found
+    uint32_t* mult;            // per record: boards found
     uint32_t* err;
 };
 
 constexpr uint32_t kTieThreads = 256;
-#ifndef NQM_TIE_STEPS
-#define NQM_TIE_STEPS 4
+#ifndef NQM_TIE_PASSES
+#define NQM_TIE_PASSES 2
 #endif
-constexpr uint32_t kTieSteps = NQM_TIE_STEPS;  // search steps per lane between two refill checks
+constexpr uint32_t kTiePasses = NQM_TIE_PASSES;  // search passes per lane between two refill checks
+#ifndef NQM_TIE_REFILL
+#define NQM_TIE_REFILL 16
+#endif
+constexpr uint32_t kTieRefill = NQM_TIE_REFILL;  // idle lanes that make a wave refill
+#ifndef NQM_TIE_FLAT
+#define NQM_TIE_FLAT 1
+#endif
+constexpr bool kTieFlat = NQM_TIE_FLAT != 0;  // TieSearch::pass_flat (selects) rather than pass (branches)
 
-// Persistent lanes: a lane takes its next record slot when it is done with
-// its record (the slots of lane g of the grid are g, g + G, g + 2 G, ...:
-// no counter that every wave of the chip would update; a wave refills once
-// 16 of its lanes idle, so the loads of a refill are shared), and lanes of
-// a wave are not held by the longest search of the wave. The search is
+// Persistent lanes: a lane takes its next record when it is done with its
+// record (the records of lane g of the grid are g, g + G, g + 2 G, ...: no
+// counter that every wave of the chip would update; a wave refills once
+// kTieRefill of its lanes idle, so the loads of a refill are shared), and lanes of a
+// wave are not held by the longest search of the wave. The search is
 // TieSearch (k4_ties.h): bottom-up from the last row but one, each row's
-// columns limited by the record's ld and rd; its open rows ({candidates
-// left, up-left diagonal}) are kept in LDS ([row][thread]). A board found is
-// written out {item, slot, path} for k4_check into the wave's board chunk
-// (one atomic per chunk); the slot's count of boards goes to mult[slot]
-// when its search ends.
+// columns limited by the record's ld and rd, one queen per pass; its rows
+// with candidates left ({f | y << 24, U, UL, UR} and RP) are kept in LDS
+// ([entry][thread]). A board found is written out {item, slot, path} for
+// k4_check into the wave's board chunk (one atomic per chunk); the record's
+// count of boards goes to mult[slot] when its search ends.
 __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
-    __shared__ uint2 stk[kK4MaxRows * kTieThreads];
+    __shared__ uint4 stk4[kTieRows * kTieThreads];
+    __shared__ uint32_t stk1[kTieRows * kTieThreads];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t n = a.n, mask = (1u << n) - 1u;
     const uint32_t gsize = gridDim.x * kTieThreads;
-    uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next slot
+    uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next record
     bool act = false;
     uint32_t slot = 0, item = 0, m = 0;
     TieSearch ts{};
-    auto slot_of = [&](uint32_t i) -> uint2& { return stk[i * kTieThreads + tid]; };
+    auto s4 = [&](uint32_t i) -> uint4& { return stk4[i * kTieThreads + tid]; };
+    auto s1 = [&](uint32_t i) -> uint32_t& { return stk1[i * kTieThreads + tid]; };
     uint64_t path = 0;  // the board found
     constexpr uint32_t kNoChunk = 0xffffffffu;
     uint32_t bchunk = kNoChunk, bcnt = 0;  // wave-uniform
     for (;;) {
         const uint64_t idle = __ballot(!act);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        const bool exhausted = __ballot(next < a.nslots) == 0;  // no lane has slots left
-        if (!exhausted && (nidle >= 16u || nidle == 64u)) {
-            const uint32_t s = next;
-            const bool mine = !act && s < a.nslots;
-            if (mine) next += gsize;
-            if (mine && (s % kRecChunk) < a.rec_counts[s / kRecChunk]) {
-                const uint4 rc = a.recs[s];
+        const bool exhausted = __ballot(next < a.nslots) == 0;  // no lane has records left
+        if (!exhausted && (nidle >= kTieRefill || nidle == 64u)) {
+            if (!act && next < a.nslots) {
+                const uint4 rc = a.recs[next];
                 const SymQ it = a.items[rc.x];
-                slot = s;
+                slot = next;
                 item = rc.x;
+                next += gsize;
                 // the drain's rd carries a marker above bit n-1
                 ts.start(n, rc.y, rc.z & mask, symq_row(it.meta), symq_b1(it.meta), symq_top_b1(it.meta) != 0u,
                          it.cols, it.ld, it.rd, it.f);
@@ -816,13 +840,13 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
         }
         if (exhausted && __ballot(act) == 0) break;
         bool found = false;
-        // up to kTieSteps steps of the lane's search before the wave's
+        // up to kTiePasses passes of the lane's search before the wave's
         // refill and board bookkeeping run again; a board found ends them
 #pragma unroll
-        for (uint32_t step = 0; step < kTieSteps; ++step) {
+        for (uint32_t p = 0; p < kTiePasses; ++p) {
             if (!act || found) continue;
             uint64_t board = 0;
-            const int r = ts.step(slot_of, &board);
+            const int r = kTieFlat ? ts.pass_flat(s4, s1, &board) : ts.pass(s4, s1, &board);
             if (r == 1) {
                 a.mult[slot] = m;
                 act = false;
@@ -855,6 +879,11 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
     }
     if (bchunk != kNoChunk && lane == 0) a.board_counts[bchunk] = bcnt;
 }
+
+#ifndef NQM_CHECK_BLOCKS
+#define NQM_CHECK_BLOCKS 4
+#endif
+constexpr uint32_t kCheckBlocksPerCu = NQM_CHECK_BLOCKS;  // workgroups of 256 per CU
 
 // One thread per board found: the rotation check (sym_check_packed on the
 // item's prefix rows and the board's path). A board behind a record with m
@@ -944,8 +973,10 @@ bool k4_supports(uint32_t n) {
     // the rd marker n + (last - R) fits in 32 bits; real ld bits (at most
     // n - 1 + (last - R)) stay below bit 31, and below the row-b2 marker,
     // which starts 31 - (b2 - R) bits up with b2 <= n - 2; the stack of
-    // last - R rows fits LDS at 16 waves per CU
-    return n + (last - R) <= 31u && n - 1u + (last - R) < 31u && n - 2u < 32u - n + R && last - R <= 9u;
+    // last - R rows fits LDS at 16 waves per CU, and a tie search's rows
+    // R+1 .. last-1 its kTieRows entries
+    return n + (last - R) <= 31u && n - 1u + (last - R) < 31u && n - 2u < 32u - n + R && last - R <= kK4MaxRows &&
+           last - R - 1u <= kTieRows;
 }
 
 // Grows b to `bytes`, keeping its first `used` bytes (stream-ordered copy).
@@ -1117,7 +1148,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
     for (int attempt = 0;; ++attempt) {
         if (nchunks * kRecChunk > 0xffffffffull) return fail(ctx, NQM_ERANGE, "too many tie records");
         if ((rc = ensure_dev(ctx, w.recs, (size_t)(nchunks + 1u) * kRecChunk * sizeof(uint4))) != NQM_OK) return rc;
-        if ((rc = ensure_dev(ctx, w.rec_counts, (size_t)(nchunks + 1u) * 4u)) != NQM_OK) return rc;
+        // counts, then offsets
+        if ((rc = ensure_dev(ctx, w.rec_counts, (size_t)(nchunks + 1u) * 8u)) != NQM_OK) return rc;
         NQM_HIP(ctx, hipMemsetAsync(d_chunk_head, 0, 16, st));  // chunk head, records, slots, boards
         NQM_HIP(ctx, hipMemsetAsync(d_head, 0, kHeadSegs * kHeadStride * 4, st));
         NQM_HIP(ctx, hipMemsetAsync(d_acc, 0, 16, st));
@@ -1134,6 +1166,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         a.next_refill = 36u;
         a.recs = static_cast<uint4*>(w.recs.ptr);
         a.rec_counts = static_cast<uint32_t*>(w.rec_counts.ptr);
+        a.chunk_off = a.rec_counts + nchunks + 1u;
         a.chunk_head = d_chunk_head;
         a.rec_total = d_rec_total;
         a.nchunks = (uint32_t)nchunks;
@@ -1158,26 +1191,29 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             continue;
         }
         NQM_HIP(ctx, hipEventElapsedTime(&search_ms, ctx->ev_start, ctx->ev_stop));
-        const uint32_t used = hc[0];
-        // ties: every record searched again, boards checked
-        const uint64_t nslots = (uint64_t)used * kRecChunk;
-        const uint32_t tie_blocks = (uint32_t)ctx->prop.multiProcessorCount * 8u;
+        const uint32_t used = hc[0], nrec = hc[1];
+        // ties: the records made dense, every record searched again, boards checked
+        const uint32_t tie_blocks = (uint32_t)ctx->prop.multiProcessorCount * 4u;  // LDS: 4 workgroups per CU
         // boards: about one per record; every research wave holds a chunk
-        const uint32_t nbchunks = 2u * (hc[1] / kBoardChunk) + tie_blocks * (kTieThreads / 64u) + 64u;
-        if ((rc = ensure_dev(ctx, w.mult, (size_t)std::max<uint64_t>(nslots, 1) * 4u)) != NQM_OK) return rc;
+        const uint32_t nbchunks = 2u * (nrec / kBoardChunk) + tie_blocks * (kTieThreads / 64u) + 64u;
+        if ((rc = ensure_dev(ctx, w.dense, (size_t)std::max<uint32_t>(nrec, 1u) * sizeof(uint4))) != NQM_OK) return rc;
+        if ((rc = ensure_dev(ctx, w.mult, (size_t)std::max<uint32_t>(nrec, 1u) * 4u)) != NQM_OK) return rc;
         if ((rc = ensure_dev(ctx, w.boards, ((size_t)nbchunks + 1u) * kBoardChunk * sizeof(uint4) +
                                                 ((size_t)nbchunks + 1u) * 4u)) != NQM_OK)
             return rc;
         uint4* const d_boards = static_cast<uint4*>(w.boards.ptr);
         uint32_t* const d_board_counts = reinterpret_cast<uint32_t*>(d_boards + ((size_t)nbchunks + 1u) * kBoardChunk);
-        if (hc[1]) {
+        if (nrec) {
+            const uint32_t* const d_counts = static_cast<const uint32_t*>(w.rec_counts.ptr);
+            hipLaunchKernelGGL(k4_rec_compact_kernel, dim3(used), dim3(256), 0, st, static_cast<const uint4*>(w.recs.ptr),
+                               d_counts, d_counts + nchunks + 1u, static_cast<uint4*>(w.dense.ptr));
+            NQM_HIP(ctx, hipGetLastError());
             K4TieArgs ta;
-            ta.recs = static_cast<const uint4*>(w.recs.ptr);
-            ta.rec_counts = static_cast<const uint32_t*>(w.rec_counts.ptr);
-            ta.nslots = (uint32_t)nslots;
+            ta.recs = static_cast<const uint4*>(w.dense.ptr);
+            ta.nslots = nrec;
             ta.items = items;
             ta.n = n;
-                ta.boards = d_boards;
+            ta.boards = d_boards;
             ta.board_counts = d_board_counts;
             ta.bchunk_head = d_bchunk_head;
             ta.nbchunks = nbchunks;
@@ -1185,7 +1221,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             ta.err = d_err;
             hipLaunchKernelGGL(k4_research_kernel, dim3(tie_blocks), dim3(kTieThreads), 0, st, ta);
             NQM_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k4_check_kernel, dim3((uint32_t)ctx->prop.multiProcessorCount * 4u), dim3(256), 0, st,
+            hipLaunchKernelGGL(k4_check_kernel, dim3((uint32_t)ctx->prop.multiProcessorCount * kCheckBlocksPerCu),
+                               dim3(256), 0, st,
                                d_boards, d_board_counts,
                                d_bchunk_head, nbchunks, static_cast<const uint32_t*>(w.mult.ptr), items, n, d_acc,
                                d_err);
@@ -1227,7 +1264,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         stats->kernel_ms = all_ms;
         stats->search_ms = search_ms;
         stats->wall_ms = now_ms() - t0;
-        stats->launches = 3u * L + 3u;
+        stats->launches = 3u * L + 4u;
         stats->blocks = blocks;
         stats->rounds = 1;
         stats->lanes = (uint64_t)blocks * kK4Threads;

@@ -28,6 +28,11 @@
 //   * the board is a tie: its last queen on endb, or bit b1 in L.
 // Bottom-up the rows with few candidates come first: about 18 placements per
 // record at N=18 (measured on the host on a sample), against 28 top-down.
+//
+// The search runs in passes, as the drain does: a pass whose row is spent
+// first goes back to the last row with candidates left (one stack entry per
+// such row, not per row), then places the next candidate. So every pass
+// places a queen, and lanes of a wave run the same few instructions.
 #pragma once
 
 #include <cstdint>
@@ -36,16 +41,20 @@
 
 namespace nqm {
 
+// Stack entries a search holds at most: rows r0+1 .. last-1 (r0 is the
+// drain's item row, ceil(n/2) - 1; 8 for n <= 19).
+constexpr uint32_t kTieRows = 8;
+
 struct TieSearch {
     // the record and its item
-    uint32_t n, last, mask, L, R, r0, b2, side, lastmask, endb, bitb1;
-    uint32_t cols0, ld0, rd0, f0;  // row r0: columns taken above it, diagonals, its candidates
+    uint32_t last, mask, L, R, r0, b2, side, bad, tie_l, endb;
+    uint32_t cols0, ld0, rd0, f0, rdk0;  // row r0: columns taken above it, diagonals, its candidates; rd0 at the last row
     // the search: current row y and its candidates left f; columns U and
     // diagonals UL, UR of the rows placed below y, as seen from row y; RP =
     // their rd bits on the last row; Ub2 / cb2 = columns of rows b2.. and
     // row b2's column (set when row b2 is placed); path = 5-bit columns of
-    // rows r0.. at bit 5 (row - r0)
-    uint32_t y, f, U, UL, UR, RP, Ub2, cb2;
+    // rows r0.. at bit 5 (row - r0); sp = stack entries
+    uint32_t y, f, U, UL, UR, RP, Ub2, cb2, sp;
     uint64_t path;
 
     __host__ __device__ inline uint32_t allowed(uint32_t yy) const {
@@ -60,7 +69,6 @@ struct TieSearch {
     __host__ __device__ inline void start(uint32_t n_, uint32_t L_, uint32_t R_, uint32_t row, uint32_t b1,
                                           bool top_b1, uint32_t cols, uint32_t ld, uint32_t rd, uint32_t cand) {
         const SymCase c = sym_case(n_, b1, 0u);
-        n = n_;
         last = c.last;
         mask = c.mask;
         L = L_;
@@ -68,36 +76,36 @@ struct TieSearch {
         r0 = row;
         b2 = c.b2;
         side = c.side;
-        lastmask = c.lastmask;
+        bad = L_ | R_ | c.lastmask;
+        tie_l = L_ & (1u << b1);
         endb = (b1 < row && !top_b1) ? c.endbit : 0xffffffffu;
-        bitb1 = 1u << b1;
         cols0 = cols;
         ld0 = ld;
         rd0 = rd;
         f0 = cand;
-        y = last - 1u;
-        U = UL = UR = RP = Ub2 = cb2 = 0u;
+        rdk0 = rd >> (c.last - row);
+        y = c.last - 1u;
+        U = UL = UR = RP = Ub2 = cb2 = sp = 0u;
         path = 0;
         f = allowed(y);
     }
 
-    // One step: back down a row, or place a queen. Returns 1 when the search
-    // is over, 2 with a board found (*board: 5-bit columns of rows r0..last,
-    // row r0 at bit 0), else 0. stk(i) is the slot of row r0 + i ({f, UL} of
-    // a row left upwards; i < last - r0).
-    template <class Stk>
-    __host__ __device__ inline int step(Stk stk, uint64_t* board) {
+    // One pass. Returns 1 when the search is over, 2 with a board found
+    // (*board: 5-bit columns of rows r0..last, row r0 at bit 0), else 0.
+    // s4(i), s1(i): stack entry i, {f | y << 24, U, UL, UR} and RP (i <
+    // kTieRows).
+    template <class S4, class S1>
+    __host__ __device__ inline int pass(S4 s4, S1 s1, uint64_t* board) {
         if (f == 0u) {
-            if (y == last - 1u) return 1;
-            ++y;
-            const uint32_t b = 1u << (uint32_t)((path >> (5u * (y - r0))) & 31u);
-            U ^= b;
-            UR = (UR >> 1) & ~b;
-            RP ^= b >> (last - y);
-            const uint2 e = stk(y - r0);
-            f = e.x;
-            UL = e.y;
-            return 0;
+            if (sp == 0u) return 1;
+            --sp;
+            const uint4 e = s4(sp);
+            f = e.x & 0xffffffu;
+            y = e.x >> 24;
+            U = e.y;
+            UL = e.z;
+            UR = e.w;
+            RP = s1(sp);
         }
         const uint32_t bit = f & (0u - f);
         f ^= bit;
@@ -107,26 +115,83 @@ struct TieSearch {
             Ub2 = U | bit;
             cb2 = bit;
         }
+        const uint32_t k = last - y;
         if (y == r0) {  // every row placed but the last: its column is the one left
             const uint32_t all = U | bit, left = mask & ~(cols0 | all);
-            const uint32_t k0 = last - r0;
-            const uint32_t rc = (rd0 >> k0) | RP | (bit >> k0);
+            const uint32_t rc = rdk0 | RP | (bit >> k);
             const uint32_t s = side & (cols0 | (all & ~Ub2));  // side columns above row b2
-            const bool ok = left != 0u && rc == R && (left & (L | R)) == 0u && (left & lastmask) == 0u &&
-                            s != 0u && (s == side || (cb2 & side) != 0u) && ((left & endb) | (L & bitb1)) != 0u;
+            const bool ok = left != 0u && rc == R && (left & bad) == 0u && s != 0u &&
+                            (s == side || (cb2 & side) != 0u) && ((left & endb) | tie_l) != 0u;
             if (ok) {
-                *board = path | ((uint64_t)__builtin_ctz(left) << (5u * (last - r0)));
+                *board = path | ((uint64_t)__builtin_ctz(left) << (5u * k));
                 return 2;
             }
             return 0;
         }
-        stk(y - r0) = make_uint2(f, UL);
+        if (f != 0u) {
+            s4(sp) = make_uint4(f | (y << 24), U, UL, UR);
+            s1(sp) = RP;
+            ++sp;
+        }
         U |= bit;
         UL = (UL | bit) >> 1;
         UR = (UR | bit) << 1;
-        RP ^= bit >> (last - y);
+        RP |= bit >> k;
         --y;
         f = allowed(y) & ~(U | UL | UR);
+        return 0;
+    }
+
+    // The same pass written as selects: every lane runs every instruction
+    // (both stack reads, the completion test and the next row's candidates),
+    // so lanes of a wave at different rows or popping and not popping issue
+    // one stream. Returns what pass() returns, with the same effects.
+    template <class S4, class S1>
+    __host__ __device__ inline int pass_flat(S4 s4, S1 s1, uint64_t* board) {
+        const bool pop = f == 0u;
+        const uint32_t top = sp ? sp - 1u : 0u;
+        const uint4 e = s4(top);
+        const uint32_t rp = s1(top);
+        if (pop && sp == 0u) return 1;
+        f = pop ? (e.x & 0xffffffu) : f;
+        y = pop ? (e.x >> 24) : y;
+        U = pop ? e.y : U;
+        UL = pop ? e.z : UL;
+        UR = pop ? e.w : UR;
+        RP = pop ? rp : RP;
+        sp = pop ? top : sp;
+        const uint32_t bit = f & (0u - f);
+        f ^= bit;
+        const uint32_t j = y - r0, k = last - y;
+        const uint32_t sh = 5u * j;
+        path = (path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
+        const bool atb2 = y == b2, at0 = j == 0u;
+        Ub2 = atb2 ? (U | bit) : Ub2;
+        cb2 = atb2 ? bit : cb2;
+        // row r0: the board's test (every row placed but the last)
+        const uint32_t all = U | bit, left = mask & ~(cols0 | all);
+        const uint32_t rc = rdk0 | RP | (bit >> k);
+        const uint32_t s = side & (cols0 | (all & ~Ub2));
+        const bool ok = at0 & (left != 0u) & (rc == R) & ((left & bad) == 0u) & (s != 0u) &
+                        ((s == side) | ((cb2 & side) != 0u)) & (((left & endb) | tie_l) != 0u);
+        // other rows: keep the row if candidates are left, go up one row
+        const bool push = !at0 && f != 0u;
+        if (push) {
+            s4(sp) = make_uint4(f | (y << 24), U, UL, UR);
+            s1(sp) = RP;
+        }
+        sp += push ? 1u : 0u;
+        U = at0 ? U : (U | bit);
+        UL = at0 ? UL : ((UL | bit) >> 1);
+        UR = at0 ? UR : ((UR | bit) << 1);
+        RP = at0 ? RP : (RP | (bit >> k));
+        y = at0 ? y : y - 1u;
+        const uint32_t fa = allowed(y) & ~(U | UL | UR);
+        f = at0 ? f : fa;
+        if (ok) {
+            *board = path | ((uint64_t)__builtin_ctz(left) << (5u * k));
+            return 2;
+        }
         return 0;
     }
 };

@@ -103,19 +103,22 @@ __host__ __device__ inline uint32_t sym_check(const SymCase& c, const Board& b) 
 // instead of searched for: row i of the 90-degree rotation has its queen in
 // column last - row(column i), of the 180-degree one in column
 // last - column(last - i), of the 270-degree one in column row(column
-// last - i). The comparisons stop where the loops of sym_check stop.
+// last - i). The comparisons stop where the loops of sym_check stop; the
+// inverse is built only for a board that reaches the 90 or 270 degree test.
 __host__ __device__ inline uint32_t sym_check_packed(const SymCase& c, uint64_t lo, uint64_t hi) {
     auto field = [](uint64_t a, uint64_t b, uint32_t k) -> uint32_t {
         return (uint32_t)((k < 12u ? a >> (5u * k) : b >> (5u * (k - 12u))) & 31u);
     };
     const uint32_t last = c.last;
     uint64_t ilo = 0, ihi = 0;  // row of each column, packed the same way
-    for (uint32_t r = 0; r <= last; ++r) {
-        const uint32_t k = field(lo, hi, r);
-        if (k < 12u)
-            ilo |= (uint64_t)r << (5u * k);
-        else
-            ihi |= (uint64_t)r << (5u * (k - 12u));
+    if (field(lo, hi, c.b2) == 0u || field(lo, hi, c.b1) == last) {
+        for (uint32_t r = 0; r <= last; ++r) {
+            const uint32_t k = field(lo, hi, r);
+            if (k < 12u)
+                ilo |= (uint64_t)r << (5u * k);
+            else
+                ihi |= (uint64_t)r << (5u * (k - 12u));
+        }
     }
     if (field(lo, hi, c.b2) == 0u) {  // 90 degrees
         uint32_t own = 1;

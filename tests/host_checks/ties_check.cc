@@ -1,5 +1,6 @@
 // Host check of TieSearch (gpu-n-queens_amd/csrc/k4_ties.h), the search
-// behind the symmetric drain's tie records, on small boards.
+// behind the symmetric drain's tie records, on small boards (its stack held
+// to kTieRows entries).
 //
 // For every non-corner item at the drain's row R (the host's restatement of
 // the item enumeration: row 0's column b1 < n-1-b1, rows below under
@@ -42,7 +43,7 @@ void pack(const uint32_t* col, uint32_t n, uint64_t& lo, uint64_t& hi) {
     }
 }
 
-int check(uint32_t n) {
+int check(uint32_t n, bool flat) {
     const uint32_t last = n - 1u, mask = (1u << n) - 1u, R = rows_done(n);
     std::vector<Item> items;
     for (uint32_t b1 = 1; 2u * b1 < last; ++b1) {
@@ -74,7 +75,7 @@ int check(uint32_t n) {
             };
         rec(1u, (1u << b1) << 1 & mask, 1u << b1, (1u << b1) >> 1, 0u);
     }
-    uint64_t records = 0, boards = 0, want_classes = 0, want_weight = 0, got_classes = 0, got_weight = 0;
+    uint64_t records = 0, boards = 0, passes = 0, want_classes = 0, want_weight = 0, got_classes = 0, got_weight = 0;
     for (const Item& it : items) {
         const SymCase c = sym_case(n, it.b1, 0u);
         const uint32_t endb = (it.b1 < it.row && !it.top_b1) ? c.endbit : 0xffffffffu, bitb1 = 1u << it.b1;
@@ -108,8 +109,16 @@ int check(uint32_t n) {
             TieSearch ts;
             ts.start(n, g.first.first, g.first.second & mask, it.row, it.b1, it.top_b1 != 0u, it.cols, it.ld, it.rd,
                      it.f);
-            uint2 slots[32];
-            auto stk = [&](uint32_t i) -> uint2& { return slots[i]; };
+            uint4 s4v[kTieRows] = {};
+            uint32_t s1v[kTieRows] = {};
+            auto s4 = [&](uint32_t i) -> uint4& {
+                if (i >= kTieRows) {
+                    std::printf("n=%u: stack entry %u\n", n, i);
+                    std::exit(1);
+                }
+                return s4v[i];
+            };
+            auto s1 = [&](uint32_t i) -> uint32_t& { return s1v[i]; };
             std::set<std::pair<uint64_t, uint64_t>> found;
             for (int steps = 0;; ++steps) {
                 if (steps > 100000000) {
@@ -117,7 +126,8 @@ int check(uint32_t n) {
                     return 1;
                 }
                 uint64_t b = 0;
-                const int s = ts.step(stk, &b);
+                const int s = flat ? ts.pass_flat(s4, s1, &b) : ts.pass(s4, s1, &b);
+                ++passes;
                 if (s == 1) break;
                 if (s == 2) {
                     for (uint32_t q = it.row; q <= last; ++q) col[q] = (uint32_t)((b >> (5u * (q - it.row))) & 31u);
@@ -138,9 +148,9 @@ int check(uint32_t n) {
             }
         }
     }
-    std::printf("n=%u items=%zu records=%llu tie boards=%llu classes=%llu weight=%llu\n", n, items.size(),
-                (unsigned long long)records, (unsigned long long)boards, (unsigned long long)got_classes,
-                (unsigned long long)got_weight);
+    std::printf("n=%u %s items=%zu records=%llu tie boards=%llu passes=%llu classes=%llu weight=%llu\n", n,
+                flat ? "pass_flat" : "pass", items.size(), (unsigned long long)records, (unsigned long long)boards, (unsigned long long)passes,
+                (unsigned long long)got_classes, (unsigned long long)got_weight);
     if (got_classes != want_classes || got_weight != want_weight || records == 0) {
         std::printf("n=%u: classes %llu / %llu, weight %llu / %llu\n", n, (unsigned long long)got_classes,
                     (unsigned long long)want_classes, (unsigned long long)got_weight, (unsigned long long)want_weight);
@@ -154,7 +164,8 @@ int check(uint32_t n) {
 int main(int argc, char** argv) {
     const uint32_t lo = argc > 1 ? (uint32_t)std::atoi(argv[1]) : 10u, hi = argc > 2 ? (uint32_t)std::atoi(argv[2]) : 13u;
     for (uint32_t n = lo; n <= hi; ++n)
-        if (check(n)) return 1;
+        for (bool flat : {false, true})
+            if (check(n, flat)) return 1;
     std::printf("ok\n");
     return 0;
 }
