@@ -6,21 +6,22 @@
 //
 //  * items: the host places the first rows, then k4_expand grows them level
 //    by level on the device to row R = ceil(n/2) - 1 (so that row 0's side
-//    rule, rows < b1 avoid the side columns, is done before the drain). A
-//    corner item whose rule "rows < b1 avoid column 1" reaches below R is
-//    grown to row b1 instead, so the drain never sees that rule.
+//    rule, rows < b1 avoid the side columns, is done before the drain).
 //
-//  * k4_drain: K2's pass plus the two row rules left below R, as selects:
+//  * k4_drain: K2's pass plus the row rules left below R, as selects:
 //      row b2 (non-corner): with one side column open, only it; with both
 //        open, nothing (the exact rule of sym_free, so the node numbers are
 //        the oracle's);
-//      last row (non-corner): no column within b1 of a corner.
+//      last row (non-corner): no column within b1 of a corner;
+//      row b1 (corner, b1 > R): column 1, kept out of avail above it, is
+//        given back.
 //    A lane does not know its row; a marker bit in the high, unused bits of
 //    its diagonals tells it. The ld marker sits 31 - (b2 - row) bits up, so
 //    it reaches bit 31 exactly when the child is at row b2 (and is shifted
 //    out after it); the rd marker sits n + (last - row) bits up and reaches
-//    bit n at the last row (rd only moves down, and never below n). Both ride
-//    along in the stack slots and in donated rows like the real bits. The
+//    bit n at the last row (rd only moves down, and never below n); a corner
+//    item's ld marker reaches bit 31 at the child at row b1. They ride along
+//    in the stack slots and in donated rows like the real bits. The
 //    item's ld is cut to n bits, so real diagonal bits stay below the markers
 //    for n <= 21; the stack of last - R rows fits LDS at 16 waves per CU for
 //    n <= 19 (k4_supports).
