@@ -782,43 +782,212 @@ struct K4TieArgs {
 };
 
 constexpr uint32_t kTieThreads = 256;
-#ifndef NQM_TIE_PASSES
-#define NQM_TIE_PASSES 2
-#endif
-constexpr uint32_t kTiePasses = NQM_TIE_PASSES;  // search passes per lane between two refill checks
 #ifndef NQM_TIE_REFILL
 #define NQM_TIE_REFILL 16
 #endif
 constexpr uint32_t kTieRefill = NQM_TIE_REFILL;  // idle lanes that make a wave refill
-#ifndef NQM_TIE_FLAT
-#define NQM_TIE_FLAT 1
+#ifndef NQM_TIE_LIM
+#define NQM_TIE_LIM 64
 #endif
-constexpr bool kTieFlat = NQM_TIE_FLAT != 0;  // TieSearch::pass_flat (selects) rather than pass (branches)
+constexpr uint32_t kTieLim = NQM_TIE_LIM;  // passes of a phase at most
+// LDS rows of the tie search: row 0 = the target of an empty stack's pop;
+// entry i of lane t at 16 t + 16 T (i + 1)
+constexpr uint32_t kTieRow = 16u * kTieThreads;
+
+// The record constants of a lane's tie search (TieSearch's, in the form the
+// pass below reads them): cols0m = the prefix's columns and every bit >= n.
+struct TieConst {
+    uint32_t L, R, cols0m, ld0, rd0, k0, jb2, side, bad, endb, tiel, rdk0;
+};
+
+// TieSearch::pass in asm (it manages the execution mask itself), two passes
+// per loop, until at most `floor` lanes are still searching or `lim` passes
+// ran. Lane registers: v[64:67] = {f, U, UL, UR} of the current row (f: its
+// candidates left), j = row - r0, q = the stack top's address, ub2 / cb2,
+// v[72:73] = the columns of the rows placed (row r0 + i at bit 5 i: rows are
+// placed from the last but one up, each shifted in at the bottom, and a pop
+// shifts out the rows it gives back), v70 = the column placed.
+// A pass:
+//   pop    a lane whose row is spent takes its stack top (v_cmpx on f == 0,
+//          borrow of q - row = an empty stack: the lane's search is over,
+//          it leaves alive for done);
+//   place  b = f & -f, f ^= b, its column shifted into the path; at row b2
+//          ub2 = U (the columns of the rows below) and cb2 = b;
+//   rows above r0's: push the row if candidates are left ({f | j << 24, U,
+//          UL, UR}), then the next row: U |= b, UL = (UL | b) >> 1, UR =
+//          (UR + b) << 1, j - 1, and its candidates (k = last - row)
+//          (L >> k) & ((R << k) | (2^k - 1)) & ~(cols0m | ld0 << j | rd0 >> j)
+//          & ~(U | UL | UR);
+//   row r0: the board's test (the last row's column is the one left, ~t
+//          with t = cols0m | U | b; its rd (UR + b) >> k0 with the prefix's
+//          equals R; the last queen is outside L, R and the corner columns;
+//          the side columns above row b2 are some, and both unless row b2
+//          took the other; a tie); a board passing it leaves alive for
+//          found with c0 = its row-r0 column and lastc = its last-row bit.
+//          Row r0's column is shifted back out of the path: the row takes
+//          its next candidate in place.
+// bitop3 truth tables index (S0 S1 S2) as a 3-bit number, S0 the high bit:
+// 0x40 = a & b & ~c, 0x10 = a & ~b & ~c, 0x0c = ~a & b, 0xba = (a & ~b) | c,
+// 0xae = (~a & b) | c.
+#define NQM_TIE_PASS(U)                                         \
+    "v_cmpx_eq_u32 vcc, 0, v64\n"                               \
+    "s_cbranch_execz .Ltp_np" #U "_%=\n"                        \
+    "v_subrev_co_u32 %[q], vcc, %[row], %[q]\n"                 \
+    "ds_read_b128 v[64:67], %[q] offset:%[row]\n"               \
+    "s_andn2_b64 %[alive], %[alive], vcc\n"                     \
+    "s_or_b64 %[done], %[done], vcc\n"                          \
+    "s_waitcnt lgkmcnt(0)\n"                                    \
+    "v_lshrrev_b32 %[t], 24, v64\n"                             \
+    "v_and_b32 v64, 0xffffff, v64\n"                            \
+    "v_sub_u32 %[t2], %[t], %[j]\n"                             \
+    "v_mul_u32_u24 %[t2], 5, %[t2]\n"                           \
+    "v_lshrrev_b64 v[72:73], %[t2], v[72:73]\n"                 \
+    "v_mov_b32 %[j], %[t]\n"                                    \
+    ".Ltp_np" #U "_%=:\n"                                       \
+    "s_mov_b64 exec, %[alive]\n"                                \
+    "s_cbranch_execz .Ltp_end_%=\n"                             \
+    "v_sub_u32 %[t], 0, v64\n"                                  \
+    "v_and_b32 %[bit], v64, %[t]\n"                             \
+    "v_xor_b32 v64, v64, %[bit]\n"                              \
+    "v_ffbl_b32 v70, %[bit]\n"                                  \
+    "v_lshlrev_b64 v[72:73], 5, v[72:73]\n"                     \
+    "v_or_b32 v72, v72, v70\n"                                  \
+    "v_cmp_eq_u32 vcc, %[j], %[jb2]\n"                          \
+    "v_cndmask_b32 %[cb2], %[cb2], %[bit], vcc\n"               \
+    "v_cndmask_b32 %[ub2], %[ub2], v65, vcc\n"                  \
+    "v_cmp_ne_u32 %[desc], 0, %[j]\n"                           \
+    "s_and_b64 exec, %[alive], %[desc]\n"                       \
+    "s_cbranch_execz .Ltp_nd" #U "_%=\n"                        \
+    "v_cmpx_ne_u32 vcc, 0, v64\n"                               \
+    "s_cbranch_execz .Ltp_npu" #U "_%=\n"                       \
+    "v_lshl_or_b32 v64, %[j], 24, v64\n"                        \
+    "ds_write_b128 %[q], v[64:67] offset:%[row]\n"              \
+    "v_add_u32 %[q], %[row], %[q]\n"                            \
+    ".Ltp_npu" #U "_%=:\n"                                      \
+    "s_and_b64 exec, %[alive], %[desc]\n"                       \
+    "v_or_b32 v65, v65, %[bit]\n"                               \
+    "v_or_b32 %[t], v66, %[bit]\n"                              \
+    "v_lshrrev_b32 v66, 1, %[t]\n"                              \
+    "v_add_lshl_u32 v67, v67, %[bit], 1\n"                      \
+    "v_add_u32 %[j], -1, %[j]\n"                                \
+    "v_sub_u32 %[k], %[k0], %[j]\n"                             \
+    "v_lshrrev_b32 %[a1], %[k], %[L]\n"                         \
+    "v_bfm_b32 %[t], %[k], 0\n"                                 \
+    "v_lshl_or_b32 %[t], %[R], %[k], %[t]\n"                    \
+    "v_lshlrev_b32 %[t2], %[j], %[ld0]\n"                       \
+    "v_lshrrev_b32 %[k], %[j], %[rd0]\n"                        \
+    "v_or3_b32 %[t2], %[cols0m], %[t2], %[k]\n"                 \
+    "v_bitop3_b32 %[a1], %[a1], %[t], %[t2] bitop3:0x40\n"      \
+    "v_or3_b32 %[t], v65, v66, v67\n"                           \
+    "v_bitop3_b32 v64, %[a1], %[t], 0 bitop3:0x10\n"            \
+    ".Ltp_nd" #U "_%=:\n"                                       \
+    "s_andn2_b64 exec, %[alive], %[desc]\n"                     \
+    "s_cbranch_execz .Ltp_nc" #U "_%=\n"                        \
+    "v_or3_b32 %[t], %[cols0m], v65, %[bit]\n"                  \
+    "v_add_u32 %[t2], v67, %[bit]\n"                            \
+    "v_lshrrev_b32 %[t2], %[k0], %[t2]\n"                       \
+    "v_or_b32 %[t2], %[rdk0], %[t2]\n"                          \
+    "v_cmp_eq_u32 %[ok], %[t2], %[R]\n"                         \
+    "v_bitop3_b32 %[t2], %[t], %[bad], 0 bitop3:0x0c\n"         \
+    "v_cmp_eq_u32 %[c1], 0, %[t2]\n"                            \
+    "s_and_b64 %[ok], %[ok], %[c1]\n"                           \
+    "v_or_b32 %[t2], %[ub2], %[cb2]\n"                          \
+    "v_or_b32 %[a1], v65, %[bit]\n"                             \
+    "v_bitop3_b32 %[t2], %[a1], %[t2], %[cols0m] bitop3:0xba\n" \
+    "v_and_b32 %[t2], %[t2], %[side]\n"                         \
+    "v_cmp_ne_u32 %[c1], 0, %[t2]\n"                            \
+    "s_and_b64 %[ok], %[ok], %[c1]\n"                           \
+    "v_and_b32 %[a1], %[cb2], %[side]\n"                        \
+    "v_or_b32 %[t2], %[t2], %[a1]\n"                            \
+    "v_cmp_eq_u32 %[c1], %[t2], %[side]\n"                      \
+    "s_and_b64 %[ok], %[ok], %[c1]\n"                           \
+    "v_bitop3_b32 %[t2], %[t], %[endb], %[tiel] bitop3:0xae\n"  \
+    "v_cmp_ne_u32 %[c1], 0, %[t2]\n"                            \
+    "s_and_b64 %[ok], %[ok], %[c1]\n"                           \
+    "s_and_b64 %[ok], %[ok], exec\n"                            \
+    "v_lshrrev_b64 v[72:73], 5, v[72:73]\n"                     \
+    "s_cmp_eq_u64 %[ok], 0\n"                                   \
+    "s_cbranch_scc1 .Ltp_nc" #U "_%=\n"                         \
+    "s_mov_b64 exec, %[ok]\n"                                   \
+    "v_not_b32 %[lastc], %[t]\n"                                \
+    "v_mov_b32 %[c0], v70\n"                                    \
+    "s_andn2_b64 %[alive], %[alive], %[ok]\n"                   \
+    "s_or_b64 %[found], %[found], %[ok]\n"                      \
+    ".Ltp_nc" #U "_%=:\n"                                       \
+    "s_mov_b64 exec, %[alive]\n"
+
+typedef uint32_t k4_v2 __attribute__((ext_vector_type(2)));
+
+// Runs the passes of the searching lanes (the lanes that call it). Returns
+// the lanes whose search ended (done) and the lanes that found a board
+// (found, with c0 and lastc).
+__device__ __forceinline__ void tie_phase(k4_v4& st, uint32_t& j, uint32_t& q, uint32_t& ub2, uint32_t& cb2,
+                                          uint64_t& path, const TieConst& c, uint32_t floor, uint32_t lim,
+                                          uint64_t& done, uint64_t& found, uint32_t& c0, uint32_t& lastc) {
+    uint32_t col = 0;
+    k4_v2 pth = {(uint32_t)path, (uint32_t)(path >> 32)};
+    uint64_t live, alive, desc, ok, c1, dn, fd;
+    uint32_t bit, t, t2, k, a1, ran, cnt;
+    asm volatile(
+        "s_mov_b64 %[live], exec\n"
+        "s_mov_b64 %[alive], exec\n"
+        "s_mov_b64 %[done], 0\n"
+        "s_mov_b64 %[found], 0\n"
+        "s_mov_b32 %[ran], 0\n"
+        ".Ltp_pass_%=:\n" NQM_TIE_PASS(0) NQM_TIE_PASS(1)
+        "s_add_u32 %[ran], %[ran], 2\n"
+        "s_bcnt1_i32_b64 %[cnt], %[alive]\n"
+        "s_cmp_le_u32 %[cnt], %[floor]\n"
+        "s_cbranch_scc1 .Ltp_end_%=\n"
+        "s_cmp_lt_u32 %[ran], %[lim]\n"
+        "s_cbranch_scc1 .Ltp_pass_%=\n"
+        ".Ltp_end_%=:\n"
+        "s_mov_b64 exec, %[live]\n"
+        "s_waitcnt lgkmcnt(0)\n"
+        : "+{v[64:67]}"(st), "+{v70}"(col), "+{v[72:73]}"(pth), [q] "+v"(q), [j] "+v"(j), [ub2] "+v"(ub2),
+          [cb2] "+v"(cb2), [c0] "+v"(c0), [lastc] "+v"(lastc), [bit] "=&v"(bit), [t] "=&v"(t), [t2] "=&v"(t2),
+          [k] "=&v"(k), [a1] "=&v"(a1), [live] "=&s"(live), [alive] "=&s"(alive), [desc] "=&s"(desc),
+          [ok] "=&s"(ok), [c1] "=&s"(c1), [done] "=&s"(dn), [found] "=&s"(fd), [ran] "=&s"(ran),
+          [cnt] "=&s"(cnt)
+        : [row] "i"(kTieRow), [floor] "s"(floor), [lim] "s"(lim), [L] "v"(c.L), [R] "v"(c.R),
+          [cols0m] "v"(c.cols0m), [ld0] "v"(c.ld0), [rd0] "v"(c.rd0), [k0] "v"(c.k0), [jb2] "v"(c.jb2),
+          [side] "v"(c.side), [bad] "v"(c.bad), [endb] "v"(c.endb), [tiel] "v"(c.tiel), [rdk0] "v"(c.rdk0)
+        : "vcc", "scc", "memory");
+    (void)col;
+    path = ((uint64_t)pth.y << 32) | pth.x;
+    done = dn;
+    found = fd;
+}
 
 // Persistent lanes: a lane takes its next record when it is done with its
 // record (the records of lane g of the grid are g, g + G, g + 2 G, ...: no
 // counter that every wave of the chip would update; a wave refills once
 // kTieRefill of its lanes idle, so the loads of a refill are shared), and lanes of a
 // wave are not held by the longest search of the wave. The search is
-// TieSearch (k4_ties.h): bottom-up from the last row but one, each row's
-// columns limited by the record's ld and rd, one queen per pass; its rows
-// with candidates left ({f | y << 24, U, UL, UR} and RP) are kept in LDS
-// ([entry][thread]). A board found is written out {item, slot, path} for
-// k4_check into the wave's board chunk (one atomic per chunk); the record's
-// count of boards goes to mult[slot] when its search ends.
+// TieSearch (k4_ties.h) pass for pass, in asm (tie_phase): bottom-up from the
+// last row but one, each row's columns limited by the record's ld and rd,
+// one queen per pass; its rows with candidates left are kept in LDS
+// ([entry][thread], 36 KB per workgroup). A board found is written out
+// {item, slot, path} for k4_check into the wave's board chunk (one atomic per
+// chunk); the record's count of boards goes to mult[slot] when its search
+// ends.
 __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
-    __shared__ uint4 stk4[kTieRows * kTieThreads];
-    __shared__ uint32_t stk1[kTieRows * kTieThreads];
+    __shared__ uint4 tie_lds[(kTieRows + 1u) * kTieThreads];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t n = a.n, mask = (1u << n) - 1u;
+    if ((uint32_t)(size_t)(k4_lds_u32*)tie_lds != 0u) {  // the stack is addressed from 0
+        if (tid == 0) atomicOr(a.err, 16u);
+        return;
+    }
+    const uint32_t n = a.n, mask = (1u << n) - 1u, last = n - 1u;
     const uint32_t gsize = gridDim.x * kTieThreads;
+    const uint32_t base = 16u * tid;
     uint32_t next = blockIdx.x * kTieThreads + tid;  // this lane's next record
     bool act = false;
     uint32_t slot = 0, item = 0, m = 0;
-    TieSearch ts{};
-    auto s4 = [&](uint32_t i) -> uint4& { return stk4[i * kTieThreads + tid]; };
-    auto s1 = [&](uint32_t i) -> uint32_t& { return stk1[i * kTieThreads + tid]; };
-    uint64_t path = 0;  // the board found
+    k4_v4 st = {0u, 0u, 0u, 0u};
+    uint32_t j = 0, q = base, ub2 = 0, cb2 = 0, c0 = 0, lastc = 0;
+    uint64_t path = 0;
+    TieConst c{};
     constexpr uint32_t kNoChunk = 0xffffffffu;
     uint32_t bchunk = kNoChunk, bcnt = 0;  // wave-uniform
     for (;;) {
@@ -833,33 +1002,40 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
                 item = rc.x;
                 next += gsize;
                 // the drain's rd carries a marker above bit n-1
+                TieSearch ts;
                 ts.start(n, rc.y, rc.z & mask, symq_row(it.meta), symq_b1(it.meta), symq_top_b1(it.meta) != 0u,
-                         it.cols, it.ld, it.rd, it.f);
+                         it.cols, it.ld, it.rd);
+                c = TieConst{ts.L, ts.R, ts.cols0 | ~mask, ts.ld0, ts.rd0, last - ts.r0, ts.b2 - ts.r0, ts.side,
+                             ts.bad, ts.endb, ts.tie_l, ts.rdk0};
+                st = k4_v4{ts.f, 0u, 0u, 0u};
+                j = ts.y - ts.r0;
+                q = base;
+                ub2 = cb2 = 0u;
+                path = 0;
                 m = 0;
                 act = true;
             }
         }
-        if (exhausted && __ballot(act) == 0) break;
-        bool found = false;
-        // up to kTiePasses passes of the lane's search before the wave's
-        // refill and board bookkeeping run again; a board found ends them
-#pragma unroll
-        for (uint32_t p = 0; p < kTiePasses; ++p) {
-            if (!act || found) continue;
-            uint64_t board = 0;
-            const int r = kTieFlat ? ts.pass_flat(s4, s1, &board) : ts.pass(s4, s1, &board);
-            if (r == 1) {
-                a.mult[slot] = m;
-                act = false;
-            } else if (r == 2) {
-                found = true;
-                ++m;
-                path = board;
-            }
+        const uint64_t actm = __ballot(act);
+        if (exhausted && actm == 0) break;
+        // the phase ends once kTieRefill of the searching lanes stopped (done
+        // or found), or after kTieLim passes
+        const uint32_t nact = (uint32_t)__popcll(actm);
+        const uint32_t floor = __builtin_amdgcn_readfirstlane(!exhausted && nact > kTieRefill ? nact - kTieRefill : 0u);
+        bool dne = false, fnd = false;
+        if (act) {
+            uint64_t dm = 0, fm = 0;
+            tie_phase(st, j, q, ub2, cb2, path, c, floor, kTieLim, dm, fm, c0, lastc);
+            dne = (dm >> lane) & 1u;
+            fnd = (fm >> lane) & 1u;
         }
-        const uint64_t fm = __ballot(found);
-        if (fm) {
-            const uint32_t nb = (uint32_t)__popcll(fm);
+        if (dne) {
+            a.mult[slot] = m;
+            act = false;
+        }
+        const uint64_t found = __ballot(fnd);
+        if (found) {
+            const uint32_t nb = (uint32_t)__popcll(found);
             if (bchunk == kNoChunk || bcnt + nb > kBoardChunk) {
                 if (bchunk != kNoChunk && lane == 0) a.board_counts[bchunk] = bcnt;
                 uint32_t ch = 0;
@@ -872,9 +1048,13 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
                 bchunk = ch;
                 bcnt = 0;
             }
-            if (found)
-                a.boards[(size_t)bchunk * kBoardChunk + bcnt + k4_rank(fm)] =
-                    make_uint4(item, slot, (uint32_t)path, (uint32_t)(path >> 32));
+            if (fnd) {
+                // rows r0+1.. from the path, row r0's column, the last row's
+                const uint64_t board = (path << 5) | c0 | ((uint64_t)__builtin_ctz(lastc) << (5u * c.k0));
+                ++m;
+                a.boards[(size_t)bchunk * kBoardChunk + bcnt + k4_rank(found)] =
+                    make_uint4(item, slot, (uint32_t)board, (uint32_t)(board >> 32));
+            }
             bcnt += nb;
         }
     }

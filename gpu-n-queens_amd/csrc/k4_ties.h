@@ -13,8 +13,9 @@
 //     carried down from row r0),
 //   * no queen already placed below attacks (U, UL, UR: columns and the two
 //     diagonals, moved up one row per row),
-// and row r0's among the item's own candidates. The last row takes the one
-// column left. The drain's own board is one such board, so the search finds
+// (at row r0 that is the item's own candidates: no rule of the non-corner
+// case applies at row r0, which lies between b1 and b2). The last row takes
+// the one column left. The drain's own board is one such board, so the search finds
 // it; every other board it completes is checked against what the record
 // says:
 //   * the last queen is not attacked: not in L nor in R (the last row's
@@ -33,6 +34,8 @@
 // first goes back to the last row with candidates left (one stack entry per
 // such row, not per row), then places the next candidate. So every pass
 // places a queen, and lanes of a wave run the same few instructions.
+// k4_research_kernel runs this pass in asm (tie_phase in k4_sym.hip), on the
+// same stack entries, with the path kept as a shift register.
 #pragma once
 
 #include <cstdint>
@@ -48,7 +51,7 @@ constexpr uint32_t kTieRows = 8;
 struct TieSearch {
     // the record and its item
     uint32_t last, mask, L, R, r0, b2, side, bad, tie_l, endb;
-    uint32_t cols0, ld0, rd0, f0, rdk0;  // row r0: columns taken above it, diagonals, its candidates; rd0 at the last row
+    uint32_t cols0, ld0, rd0, rdk0;  // row r0: columns taken above it, its diagonals; rd0 at the last row
     // the search: current row y and its candidates left f; columns U and
     // diagonals UL, UR of the rows placed below y, as seen from row y; RP =
     // their rd bits on the last row; Ub2 / cb2 = columns of rows b2.. and
@@ -59,15 +62,13 @@ struct TieSearch {
 
     __host__ __device__ inline uint32_t allowed(uint32_t yy) const {
         const uint32_t k = last - yy, j = yy - r0;
-        uint32_t a = (L >> k) & ((R << k) | ((1u << k) - 1u)) & ~(cols0 | (ld0 << j) | (rd0 >> j)) & mask;
-        if (yy == r0) a &= f0;
-        return a;
+        return (L >> k) & ((R << k) | ((1u << k) - 1u)) & ~(cols0 | (ld0 << j) | (rd0 >> j)) & mask;
     }
 
     // n, the record's ld and rd (rd without the drain's marker), and the
-    // item's row, b1, top_b1 flag, columns, diagonals and candidates
+    // item's row, b1, top_b1 flag, columns and diagonals
     __host__ __device__ inline void start(uint32_t n_, uint32_t L_, uint32_t R_, uint32_t row, uint32_t b1,
-                                          bool top_b1, uint32_t cols, uint32_t ld, uint32_t rd, uint32_t cand) {
+                                          bool top_b1, uint32_t cols, uint32_t ld, uint32_t rd) {
         const SymCase c = sym_case(n_, b1, 0u);
         last = c.last;
         mask = c.mask;
@@ -82,7 +83,6 @@ struct TieSearch {
         cols0 = cols;
         ld0 = ld;
         rd0 = rd;
-        f0 = cand;
         rdk0 = rd >> (c.last - row);
         y = c.last - 1u;
         U = UL = UR = RP = Ub2 = cb2 = sp = 0u;
@@ -139,59 +139,6 @@ struct TieSearch {
         RP |= bit >> k;
         --y;
         f = allowed(y) & ~(U | UL | UR);
-        return 0;
-    }
-
-    // The same pass written as selects: every lane runs every instruction
-    // (both stack reads, the completion test and the next row's candidates),
-    // so lanes of a wave at different rows or popping and not popping issue
-    // one stream. Returns what pass() returns, with the same effects.
-    template <class S4, class S1>
-    __host__ __device__ inline int pass_flat(S4 s4, S1 s1, uint64_t* board) {
-        const bool pop = f == 0u;
-        const uint32_t top = sp ? sp - 1u : 0u;
-        const uint4 e = s4(top);
-        const uint32_t rp = s1(top);
-        if (pop && sp == 0u) return 1;
-        f = pop ? (e.x & 0xffffffu) : f;
-        y = pop ? (e.x >> 24) : y;
-        U = pop ? e.y : U;
-        UL = pop ? e.z : UL;
-        UR = pop ? e.w : UR;
-        RP = pop ? rp : RP;
-        sp = pop ? top : sp;
-        const uint32_t bit = f & (0u - f);
-        f ^= bit;
-        const uint32_t j = y - r0, k = last - y;
-        const uint32_t sh = 5u * j;
-        path = (path & ~(31ull << sh)) | ((uint64_t)__builtin_ctz(bit) << sh);
-        const bool atb2 = y == b2, at0 = j == 0u;
-        Ub2 = atb2 ? (U | bit) : Ub2;
-        cb2 = atb2 ? bit : cb2;
-        // row r0: the board's test (every row placed but the last)
-        const uint32_t all = U | bit, left = mask & ~(cols0 | all);
-        const uint32_t rc = rdk0 | RP | (bit >> k);
-        const uint32_t s = side & (cols0 | (all & ~Ub2));
-        const bool ok = at0 & (left != 0u) & (rc == R) & ((left & bad) == 0u) & (s != 0u) &
-                        ((s == side) | ((cb2 & side) != 0u)) & (((left & endb) | tie_l) != 0u);
-        // other rows: keep the row if candidates are left, go up one row
-        const bool push = !at0 && f != 0u;
-        if (push) {
-            s4(sp) = make_uint4(f | (y << 24), U, UL, UR);
-            s1(sp) = RP;
-        }
-        sp += push ? 1u : 0u;
-        U = at0 ? U : (U | bit);
-        UL = at0 ? UL : ((UL | bit) >> 1);
-        UR = at0 ? UR : ((UR | bit) << 1);
-        RP = at0 ? RP : (RP | (bit >> k));
-        y = at0 ? y : y - 1u;
-        const uint32_t fa = allowed(y) & ~(U | UL | UR);
-        f = at0 ? f : fa;
-        if (ok) {
-            *board = path | ((uint64_t)__builtin_ctz(left) << (5u * k));
-            return 2;
-        }
         return 0;
     }
 };

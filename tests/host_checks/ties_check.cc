@@ -43,7 +43,7 @@ void pack(const uint32_t* col, uint32_t n, uint64_t& lo, uint64_t& hi) {
     }
 }
 
-int check(uint32_t n, bool flat) {
+int check(uint32_t n) {
     const uint32_t last = n - 1u, mask = (1u << n) - 1u, R = rows_done(n);
     std::vector<Item> items;
     for (uint32_t b1 = 1; 2u * b1 < last; ++b1) {
@@ -107,8 +107,7 @@ int check(uint32_t n, bool flat) {
         for (const auto& g : groups) {
             ++records;
             TieSearch ts;
-            ts.start(n, g.first.first, g.first.second & mask, it.row, it.b1, it.top_b1 != 0u, it.cols, it.ld, it.rd,
-                     it.f);
+            ts.start(n, g.first.first, g.first.second & mask, it.row, it.b1, it.top_b1 != 0u, it.cols, it.ld, it.rd);
             uint4 s4v[kTieRows] = {};
             uint32_t s1v[kTieRows] = {};
             auto s4 = [&](uint32_t i) -> uint4& {
@@ -126,7 +125,7 @@ int check(uint32_t n, bool flat) {
                     return 1;
                 }
                 uint64_t b = 0;
-                const int s = flat ? ts.pass_flat(s4, s1, &b) : ts.pass(s4, s1, &b);
+                const int s = ts.pass(s4, s1, &b);
                 ++passes;
                 if (s == 1) break;
                 if (s == 2) {
@@ -148,8 +147,8 @@ int check(uint32_t n, bool flat) {
             }
         }
     }
-    std::printf("n=%u %s items=%zu records=%llu tie boards=%llu passes=%llu classes=%llu weight=%llu\n", n,
-                flat ? "pass_flat" : "pass", items.size(), (unsigned long long)records, (unsigned long long)boards, (unsigned long long)passes,
+    std::printf("n=%u items=%zu records=%llu tie boards=%llu passes=%llu classes=%llu weight=%llu\n", n,
+                items.size(), (unsigned long long)records, (unsigned long long)boards, (unsigned long long)passes,
                 (unsigned long long)got_classes, (unsigned long long)got_weight);
     if (got_classes != want_classes || got_weight != want_weight || records == 0) {
         std::printf("n=%u: classes %llu / %llu, weight %llu / %llu\n", n, (unsigned long long)got_classes,
@@ -164,8 +163,7 @@ int check(uint32_t n, bool flat) {
 int main(int argc, char** argv) {
     const uint32_t lo = argc > 1 ? (uint32_t)std::atoi(argv[1]) : 10u, hi = argc > 2 ? (uint32_t)std::atoi(argv[2]) : 13u;
     for (uint32_t n = lo; n <= hi; ++n)
-        for (bool flat : {false, true})
-            if (check(n, flat)) return 1;
+        if (check(n)) return 1;
     std::printf("ok\n");
     return 0;
 }
