@@ -343,9 +343,10 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
 //          marker of a corner item), t2 = bit n of rd' (the last-row
 //          marker), avail' = ((avail ^ b) & ~(lm & t2)) | (amask & t1) (a
 //          corner item gets column 1 back at row b1), and at row b2 the side
-//          rule from the open side columns of avail' & sidel (sidel = 0 for
-//          a corner item): m0 / mt = bit 0 / n-1 as 0 or ~0, blocked = side
-//          columns where both are open, all others where either is; the
+//          rule from the open side columns: m0 / mt = bits o0 / on of avail'
+//          as 0 or ~0 (o0 = 0 and on = n-1; 31 for a corner item, whose
+//          avail' never has bit 31), blocked = side columns where both are
+//          open, all others where either is; the
 //          free set is avail' & ~ld' & ~(rd' | (blocked & t1)). The
 //          solution and tie masks come last, and only in a pass where some
 //          lane completed a board (most passes complete none).
@@ -368,8 +369,9 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_mov_b32 %[bitb1], v105\n"                           \
     "v_mov_b32 %[endb], v106\n"                            \
     "v_mov_b32 %[amask], v107\n"                           \
-    "v_mov_b32 %[sidel], v108\n"                           \
-    "v_mov_b32 v96, v109\n"                                \
+    "v_mov_b32 %[o0], v108\n"                              \
+    "v_mov_b32 %[on], v109\n"                              \
+    "v_mov_b32 v96, v110\n"                                \
     "v_add_u32 %[q], %[row], %[q]\n"                       \
     "s_andn2_b64 %[hasnext], %[hasnext], exec\n"           \
     "s_andn2_b64 vcc, vcc, exec\n"                         \
@@ -399,9 +401,8 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
     "v_bitop3_b32 " NA ", " CA ", %[b], %[t2v] bitop3:0x14\n" \
     "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"      \
-    "v_and_b32 %[m0], " NA ", %[sidel]\n"                 \
-    "v_bfe_i32 %[mt], %[m0], %[snm1], 1\n"                \
-    "v_bfe_i32 %[m0], %[m0], 0, 1\n"                      \
+    "v_bfe_i32 %[mt], " NA ", %[on], 1\n"                  \
+    "v_bfe_i32 %[m0], " NA ", %[o0], 1\n"                  \
     "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
     "v_bitop3_b32 %[t1], " NR ", %[mt], %[t1] bitop3:0xf8\n" \
     "v_bitop3_b32 " NX ", " NA ", " NL ", %[t1] bitop3:0x10\n" \
@@ -436,9 +437,10 @@ typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
 
 typedef uint32_t k4_v8 __attribute__((ext_vector_type(8)));
 
-// The rule constants of a lane's piece (see take() in k4_drain_kernel).
+// The rule constants of a lane's piece (see load_piece in k4_drain_kernel):
+// o0 / on = the bits of avail that are the side columns (31: none).
 struct K4Rules {
-    uint32_t lm, bitb1, endb, amask, sidel, item;
+    uint32_t lm, bitb1, endb, amask, o0, on, item;
 };
 
 // Runs passes until at most floor_live lanes hold a piece, at most nfloor
@@ -449,15 +451,15 @@ struct K4Rules {
 // and the placements (pnodes) of the wave.
 __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, const uint4& nst, const K4Rules& nr,
                                          uint64_t& hasnext, int32_t nfloor, uint32_t& cnt,
-                                         uint64_t recbase, uint32_t sn,
-                                         uint32_t snm1, uint32_t sside, uint32_t floor_live, uint32_t lim32,
+                                         uint64_t recbase, uint32_t sn, uint32_t sside, uint32_t floor_live,
+                                         uint32_t lim32,
                                          uint32_t& psols, uint32_t& pnodes, uint32_t& ran) {
     k4_v4 s4 = {st.x, st.y, st.z, st.w};
     k4_v4 s5;
     k4_v4 rec = {rl.item, 0u, 0u, 0u};
     const k4_v4 n4 = {nst.x, nst.y, nst.z, nst.w};
-    const k4_v8 n8 = {nr.lm, nr.bitb1, nr.endb, nr.amask, nr.sidel, nr.item, 0u, 0u};
-    uint32_t lm = rl.lm, bitb1 = rl.bitb1, endb = rl.endb, amask = rl.amask, sidel = rl.sidel;
+    const k4_v8 n8 = {nr.lm, nr.bitb1, nr.endb, nr.amask, nr.o0, nr.on, nr.item, 0u};
+    uint32_t lm = rl.lm, bitb1 = rl.bitb1, endb = rl.endb, amask = rl.amask, o0 = rl.o0, on = rl.on;
     uint64_t live, alive, sm, se;
     uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
     uint32_t sols = 0, nodes = 0;
@@ -490,13 +492,13 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, co
           [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live),
           [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
           [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c), [lm] "+v"(lm), [bitb1] "+v"(bitb1),
-          [endb] "+v"(endb), [amask] "+v"(amask), [sidel] "+v"(sidel), [hasnext] "+s"(hasnext)
+          [endb] "+v"(endb), [amask] "+v"(amask), [o0] "+v"(o0), [on] "+v"(on), [hasnext] "+s"(hasnext)
         : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [sn] "s"(sn),
-          [snm1] "s"(snm1), [sside] "s"(sside), [recbase] "s"(recbase), [nfloor] "s"(nfloor),
+          [sside] "s"(sside), [recbase] "s"(recbase), [nfloor] "s"(nfloor),
           "{v[100:103]}"(n4), "{v[104:111]}"(n8)
         : "vcc", "scc", "memory");
     (void)s5;
-    rl = K4Rules{lm, bitb1, endb, amask, sidel, rec.x};
+    rl = K4Rules{lm, bitb1, endb, amask, o0, on, rec.x};
     st = make_uint4(s4.x, s4.y, s4.z, s4.w);
     cnt = c;
     psols = sols;
@@ -561,7 +563,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         const SymQ it = a.items[idx];
         const uint32_t r = symq_row(it.meta), b1 = symq_b1(it.meta);
         uint32_t ld = it.ld, rd = it.rd, avail = ~it.cols & a.mask;
-        pr = K4Rules{0u, 0u, 0u, 0u, 0u, idx};
+        pr = K4Rules{0u, 0u, 0u, 0u, 31u, 31u, idx};
         if (symq_corner(it.meta)) {
             if (b1 > r) {  // rows below b1 avoid column 1: out of avail until the child at row b1
                 avail &= ~2u;
@@ -569,7 +571,8 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                 pr.amask = 2u;
             }
         } else {
-            pr.sidel = a.side;
+            pr.o0 = 0u;
+            pr.on = a.n - 1u;
             const SymCase c = sym_case(a.n, b1, 0u);
             if (c.b2 > r) ld |= 1u << (31u - (c.b2 - r));  // reaches bit 31 at child row b2
             rd |= 1u << (a.n + (a.last - r));              // reaches bit n at the last row
@@ -657,7 +660,8 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                     const uint32_t dl = recv ? wscratch[ir] : lane;
                     const K4Rules d{(uint32_t)__shfl(rl.lm, dl, 64),    (uint32_t)__shfl(rl.bitb1, dl, 64),
                                     (uint32_t)__shfl(rl.endb, dl, 64),  (uint32_t)__shfl(rl.amask, dl, 64),
-                                    (uint32_t)__shfl(rl.sidel, dl, 64), (uint32_t)__shfl(rl.item, dl, 64)};
+                                    (uint32_t)__shfl(rl.o0, dl, 64),    (uint32_t)__shfl(rl.on, dl, 64),
+                                    (uint32_t)__shfl(rl.item, dl, 64)};
                     if (recv) {
                         st = k4_slot_load(base - 16u * lane + 16u * dl + row);  // the donor's slot 0
                         rl = d;
@@ -714,7 +718,7 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
         if (holds()) {
             const uint64_t had = hasnext;
-            k4_phase(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.n - 1u, a.side, floor_live, limit,
+            k4_phase(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
                      psols, pnodes, ran);
             // the phase cleared the bits of the lanes that started their next item
             if (((had & ~hasnext) >> lane) & 1u) holdnext = false;
