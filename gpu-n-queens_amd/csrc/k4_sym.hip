@@ -764,12 +764,19 @@ constexpr uint32_t kBoardChunk = 1024;  // boards per chunk of a wave
 
 // Dense records: chunk ch's records go to [chunk_off[ch], + rec_counts[ch]),
 // so the search below takes no empty slot (a wave's last chunk is part full).
+// One workgroup per chunk; those past the chunks handed out return. The list
+// holds cap records (more means the count changed: the host fails it).
 __global__ void __launch_bounds__(256) k4_rec_compact_kernel(const uint4* __restrict__ recs,
                                                             const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ offs,
-                                                            uint4* __restrict__ dense) {
-    const uint32_t ch = blockIdx.x, cnt = counts[ch], off = offs[ch];
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256u) dense[off + i] = recs[(size_t)ch * kRecChunk + i];
+                                                            const uint32_t* __restrict__ chunk_head,
+                                                            uint32_t nchunks, uint4* __restrict__ dense,
+                                                            uint32_t cap) {
+    const uint32_t ch = blockIdx.x;
+    if (ch >= min(*chunk_head, nchunks)) return;
+    const uint32_t cnt = counts[ch], off = offs[ch];
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256u)
+        if (off + i < cap) dense[off + i] = recs[(size_t)ch * kRecChunk + i];
 }
 
 struct K4TieArgs {
@@ -1250,9 +1257,12 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
                 return rc;
     }
 
+    // the host's items go up from pinned memory (an asynchronous copy)
+    if ((rc = ensure_host(ctx, w.h_items, std::max<size_t>(hq.size(), 1) * sizeof(SymQ))) != NQM_OK) return rc;
+    std::copy(hq.begin(), hq.end(), static_cast<SymQ*>(w.h_items.ptr));
     NQM_HIP(ctx, hipEventRecord(ctx->ev_x0, st));
     NQM_HIP(ctx, hipMemsetAsync(d_cnt, 0, 4096, st));
-    NQM_HIP(ctx, hipMemcpyAsync(w.items.ptr, hq.data(), hq.size() * sizeof(SymQ), hipMemcpyHostToDevice, st));
+    NQM_HIP(ctx, hipMemcpyAsync(w.items.ptr, w.h_items.ptr, hq.size() * sizeof(SymQ), hipMemcpyHostToDevice, st));
     // the final queue starts with the host's final items
     if (!known && (rc = grow_keep(ctx, w.fin, 0, (size_t)std::max<uint32_t>(host_fin, 1) * sizeof(SymQ))) != NQM_OK)
         return rc;
@@ -1311,6 +1321,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         w.level_in = level_in;
         w.level_fin = level_fin;
         w.chunks = 0;
+        w.nrec_known = false;
     }
     const uint32_t nfin = (uint32_t)nfin64;
     const SymQ* const items = static_cast<const SymQ*>(w.fin.ptr);
@@ -1363,19 +1374,23 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             NQM_HIP(ctx, hipGetLastError());
         }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_stop, st));
-        uint32_t hc[2] = {0, 0}, err = 0;  // chunks used, records written
-        NQM_HIP(ctx, hipMemcpyAsync(hc, d_chunk_head, 8, hipMemcpyDeviceToHost, st));
-        NQM_HIP(ctx, hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, st));
-        NQM_HIP(ctx, hipStreamSynchronize(st));
-        if (err & 1u) return fail(ctx, NQM_EDEVICE, "symmetric expansion overflowed its counted size");
-        if (err & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
-        if (err & 64u) {
-            if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
-            nchunks *= 4u;
-            NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
-            continue;
+        // the tie stage's sizes: from the last count of this n (its records
+        // are the same), else read back here
+        const bool rec_known = w.nrec_known;
+        uint32_t hc[2] = {(uint32_t)nchunks, w.nrec}, err = 0;  // chunks used (a bound when known), records
+        if (!rec_known) {
+            NQM_HIP(ctx, hipMemcpyAsync(hc, d_chunk_head, 8, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipMemcpyAsync(&err, d_err, 4, hipMemcpyDeviceToHost, st));
+            NQM_HIP(ctx, hipStreamSynchronize(st));
+            if (err & 1u) return fail(ctx, NQM_EDEVICE, "symmetric expansion overflowed its counted size");
+            if (err & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
+            if (err & 64u) {
+                if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
+                nchunks *= 4u;
+                NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
+                continue;
+            }
         }
-        NQM_HIP(ctx, hipEventElapsedTime(&search_ms, ctx->ev_start, ctx->ev_stop));
         const uint32_t used = hc[0], nrec = hc[1];
         // ties: the records made dense, every record searched again, boards checked
         const uint32_t tie_blocks = (uint32_t)ctx->prop.multiProcessorCount * 4u;  // LDS: 4 workgroups per CU
@@ -1391,7 +1406,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         if (nrec) {
             const uint32_t* const d_counts = static_cast<const uint32_t*>(w.rec_counts.ptr);
             hipLaunchKernelGGL(k4_rec_compact_kernel, dim3(used), dim3(256), 0, st, static_cast<const uint4*>(w.recs.ptr),
-                               d_counts, d_counts + nchunks + 1u, static_cast<uint4*>(w.dense.ptr));
+                               d_counts, d_counts + nchunks + 1u, d_chunk_head, (uint32_t)nchunks,
+                               static_cast<uint4*>(w.dense.ptr), nrec);
             NQM_HIP(ctx, hipGetLastError());
             K4TieArgs ta;
             ta.recs = static_cast<const uint4*>(w.dense.ptr);
@@ -1417,13 +1433,27 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         std::vector<uint64_t> wp((size_t)nwaves * 2u);
         uint64_t hacc[2] = {};
         unsigned long long hx = 0;
-        uint32_t err2 = 0;
+        uint32_t err2 = 0, hc2[2] = {0, 0};
         NQM_HIP(ctx, hipMemcpyAsync(wp.data(), w.wave.ptr, wp.size() * 8u, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipMemcpyAsync(hacc, d_acc, sizeof(hacc), hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipMemcpyAsync(&hx, d_xnodes, 8, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipMemcpyAsync(&err2, d_err, 4, hipMemcpyDeviceToHost, st));
+        NQM_HIP(ctx, hipMemcpyAsync(hc2, d_chunk_head, 8, hipMemcpyDeviceToHost, st));
         NQM_HIP(ctx, hipStreamSynchronize(st));
         NQM_HIP(ctx, hipEventElapsedTime(&all_ms, ctx->ev_x0, ctx->ev_x1));
+        NQM_HIP(ctx, hipEventElapsedTime(&search_ms, ctx->ev_start, ctx->ev_stop));
+        if (rec_known) {  // the drain's checks, made here
+            if (err2 & 1u) return fail(ctx, NQM_EDEVICE, "symmetric expansion overflowed its counted size");
+            if (err2 & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
+            if (err2 & 64u) {  // out of record chunks: the tie stage ran on partial records; again with more
+                if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
+                nchunks *= 4u;
+                w.nrec_known = false;
+                NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
+                continue;
+            }
+            if (hc2[1] != nrec) return fail(ctx, NQM_EDEVICE, "tie records %u, %u at the last count", hc2[1], nrec);
+        }
         if (err2 & 128u) return fail(ctx, NQM_ERANGE, "tie boards exceed %u chunks", nbchunks);
         if (err2 & 256u) return fail(ctx, NQM_ERANGE, "more than %u tie boards behind one record", kMaxMult);
         // every board was counted lcm / m times m (its m records)
@@ -1436,7 +1466,9 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         tie_u = hs[0];
         tie_t = hs[1];
         xnodes = hx;
-        w.chunks = used;
+        w.chunks = hc2[0];
+        w.nrec = hc2[1];
+        w.nrec_known = true;
         break;
     }
     w.sizes_n = n;  // the sizes (and the chunk count) of this n are now known

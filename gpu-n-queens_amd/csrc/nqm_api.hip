@@ -225,6 +225,7 @@ void nqm_destroy(nqm_ctx* ctx) {
     nqm::FsWork& fw = ctx->fs;
     for (nqm::DevBuf* d : {&fw.ctr, &fw.lvl[0], &fw.lvl[1], &fw.wave}) free_dev(*d);
     free_host(ctx->h_part);
+    free_host(sw.h_items);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     if (ctx->ev_x0) (void)hipEventDestroy(ctx->ev_x0);

@@ -63,6 +63,9 @@ struct SymWork {
     uint32_t host_fin = 0;                        // items the host puts in the final queue
     std::vector<uint32_t> level_in, level_fin;    // per level: pending items in, final items out
     uint64_t chunks = 0;                          // record chunks the last count of sizes_n used
+    uint32_t nrec = 0;                            // and its tie records (the same every count of n)
+    bool nrec_known = false;
+    HostBuf h_items;                              // pinned staging of the host's items
 };
 
 // Device buffers of the full-sweep count (k5_fullsweep.hip), kept across

@@ -64,3 +64,13 @@ def test_drain_rejects_sizes_outside_its_range(ctx):
     for n in (9, 20, 21, 22):
         rc = ctx.lib.nqm_count_symmetric_path(ctx.handle, n, nqm.SYM_DRAIN, None, None, None)
         assert rc == nqm.EUNSUPPORTED, n
+
+
+def test_drain_repeated_and_alternating_sizes(ctx):
+    # a second count of an n takes the tie stage's sizes from the first (no
+    # read-back between the drain and the tie stage); another n in between
+    # resets them
+    want = {n: oracle.sym_count(n)[:2] for n in (14, 15)}
+    for n in (14, 14, 15, 14, 15, 15):
+        u, t, _ = ctx.count_symmetric(n, nqm.SYM_DRAIN)
+        assert (u, t) == want[n], n
