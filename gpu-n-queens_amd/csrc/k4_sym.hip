@@ -1073,19 +1073,21 @@ __global__ void __launch_bounds__(kTieThreads) k4_research_kernel(K4TieArgs a) {
 }
 
 #ifndef NQM_CHECK_BLOCKS
-#define NQM_CHECK_BLOCKS 4
+#define NQM_CHECK_BLOCKS 16
 #endif
 constexpr uint32_t kCheckBlocksPerCu = NQM_CHECK_BLOCKS;  // workgroups of 256 per CU
 
 // One thread per board found: the rotation check (sym_check_packed on the
 // item's prefix rows and the board's path). A board behind a record with m
 // boards is counted m times (once per copy of the record), so it adds
-// lcm(1..16) / m: acc[0] = classes x lcm, acc[1] = weight x lcm.
+// lcm(1..16) / m; each workgroup writes its sums (classes x lcm, weight x
+// lcm) to part[block], k4_sum_kernel adds them up (no same-address atomics).
 __global__ void __launch_bounds__(256)
 k4_check_kernel(const uint4* __restrict__ boards, const uint32_t* __restrict__ board_counts,
                 const uint32_t* __restrict__ bchunk_head, uint32_t nbchunks, const uint32_t* __restrict__ mult,
-                const SymQ* __restrict__ items, uint32_t n, unsigned long long* __restrict__ acc,
+                const SymQ* __restrict__ items, uint32_t n, ulonglong2* __restrict__ part,
                 uint32_t* __restrict__ err) {
+    __shared__ ulonglong2 wsum[4];
     const uint32_t nch = min(*bchunk_head, nbchunks);  // past the cap: err bit 128, the count is void
     uint64_t uq = 0, tot = 0;
     for (uint32_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
@@ -1113,9 +1115,34 @@ k4_check_kernel(const uint4* __restrict__ boards, const uint32_t* __restrict__ b
     }
     uq = k4_wave_sum64(uq);
     tot = k4_wave_sum64(tot);
-    if ((threadIdx.x & 63u) == 0u && (uq | tot)) {
-        atomicAdd(&acc[0], (unsigned long long)uq);
-        atomicAdd(&acc[1], (unsigned long long)tot);
+    if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = make_ulonglong2(uq, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ulonglong2 b = wsum[0];
+        for (uint32_t i = 1; i < 4u; ++i) {
+            b.x += wsum[i].x;
+            b.y += wsum[i].y;
+        }
+        part[blockIdx.x] = b;
+    }
+}
+
+// acc = the sum of the check's per-workgroup sums (one workgroup).
+__global__ void __launch_bounds__(256) k4_sum_kernel(const ulonglong2* __restrict__ part, uint32_t count,
+                                                    unsigned long long* __restrict__ acc) {
+    __shared__ ulonglong2 wsum[4];
+    uint64_t a = 0, b = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += 256u) {
+        a += part[i].x;
+        b += part[i].y;
+    }
+    a = k4_wave_sum64(a);
+    b = k4_wave_sum64(b);
+    if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = make_ulonglong2(a, b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        acc[0] = wsum[0].x + wsum[1].x + wsum[2].x + wsum[3].x;
+        acc[1] = wsum[0].y + wsum[1].y + wsum[2].y + wsum[3].y;
     }
 }
 
@@ -1334,6 +1361,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
     const uint32_t blocks = (uint32_t)ctx->prop.multiProcessorCount * per_cu;
     const uint32_t nwaves = blocks * (kK4Threads / 64u);
     if ((rc = ensure_dev(ctx, w.wave, (size_t)nwaves * 16u)) != NQM_OK) return rc;
+    if ((rc = ensure_dev(ctx, w.cpart, (size_t)ctx->prop.multiProcessorCount * kCheckBlocksPerCu * 16u)) != NQM_OK)
+        return rc;
 
     // record chunks: as many as the last count of this n used, else an
     // estimate from the queue (a run that runs out sets err bit 64 and is
@@ -1422,11 +1451,13 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             ta.err = d_err;
             hipLaunchKernelGGL(k4_research_kernel, dim3(tie_blocks), dim3(kTieThreads), 0, st, ta);
             NQM_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k4_check_kernel, dim3((uint32_t)ctx->prop.multiProcessorCount * kCheckBlocksPerCu),
-                               dim3(256), 0, st,
-                               d_boards, d_board_counts,
-                               d_bchunk_head, nbchunks, static_cast<const uint32_t*>(w.mult.ptr), items, n, d_acc,
+            const uint32_t check_blocks = (uint32_t)ctx->prop.multiProcessorCount * kCheckBlocksPerCu;
+            ulonglong2* const d_cpart = static_cast<ulonglong2*>(w.cpart.ptr);
+            hipLaunchKernelGGL(k4_check_kernel, dim3(check_blocks), dim3(256), 0, st, d_boards, d_board_counts,
+                               d_bchunk_head, nbchunks, static_cast<const uint32_t*>(w.mult.ptr), items, n, d_cpart,
                                d_err);
+            NQM_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k4_sum_kernel, dim3(1), dim3(256), 0, st, d_cpart, check_blocks, d_acc);
             NQM_HIP(ctx, hipGetLastError());
         }
         NQM_HIP(ctx, hipEventRecord(ctx->ev_x1, st));
@@ -1481,7 +1512,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
         stats->kernel_ms = all_ms;
         stats->search_ms = search_ms;
         stats->wall_ms = now_ms() - t0;
-        stats->launches = 3u * L + 4u;
+        stats->launches = 3u * L + 5u;
         stats->blocks = blocks;
         stats->rounds = 1;
         stats->lanes = (uint64_t)blocks * kK4Threads;

@@ -220,7 +220,7 @@ void nqm_destroy(nqm_ctx* ctx) {
         free_dev(*d);
     nqm::SymWork& sw = ctx->sym;
     for (nqm::DevBuf* d : {&sw.ctr, &sw.items, &sw.fin, &sw.pend[0], &sw.pend[1], &sw.xpart, &sw.recs, &sw.rec_counts,
-                           &sw.dense, &sw.boards, &sw.mult, &sw.wave})
+                           &sw.dense, &sw.boards, &sw.mult, &sw.wave, &sw.cpart})
         free_dev(*d);
     nqm::FsWork& fw = ctx->fs;
     for (nqm::DevBuf* d : {&fw.ctr, &fw.lvl[0], &fw.lvl[1], &fw.wave}) free_dev(*d);

@@ -58,7 +58,7 @@ void drop_plan(GraphPlan& plan);
 // tree depends on n only, so a repeated n launches the levels without
 // counting them first or waiting for the host).
 struct SymWork {
-    DevBuf ctr, items, fin, pend[2], xpart, recs, rec_counts, dense, boards, mult, wave;
+    DevBuf ctr, items, fin, pend[2], xpart, recs, rec_counts, dense, boards, mult, wave, cpart;
     uint32_t sizes_n = 0;                         // 0: no sizes kept
     uint32_t host_fin = 0;                        // items the host puts in the final queue
     std::vector<uint32_t> level_in, level_fin;    // per level: pending items in, final items out
