@@ -230,41 +230,61 @@ k4_expand_scan_kernel(uint4* __restrict__ part, uint32_t nblocks, uint32_t* __re
 // Block b writes its children: final ones at fin_q[fin_at + offset] (the
 // final queue already holds fin_at items), next ones at next_q[offset].
 // Offsets past a capacity set err bit 1 (the level was not the counted one).
+// All children of an item go to one queue (they share a row). The block
+// stages its items and, per output, the item and column in LDS; then thread k
+// writes outputs k, k + 256, ...: consecutive threads, consecutive items.
+constexpr uint32_t kK4MaxChildren = 20;  // columns of a row (n <= 19) and one to spare
 __global__ void __launch_bounds__(256)
 k4_expand_write_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, const uint4* __restrict__ part,
                        SymQ* __restrict__ fin_q, uint32_t fin_at, uint32_t fin_cap, SymQ* __restrict__ next_q,
                        uint32_t next_cap, uint32_t* __restrict__ err) {
     __shared__ uint2 wsum[4];
+    __shared__ SymQ s_item[256];
+    __shared__ uint8_t s_map[256u * kK4MaxChildren];  // output -> item of the block
+    __shared__ uint8_t s_col[256u * kK4MaxChildren];  // output -> column
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     SymQ it{};
-    uint32_t nf = 0, nn = 0;
+    uint32_t live = 0, nf = 0, nn = 0;
+    bool to_fin = false;
     if (i < count) {
         it = in[i];
-        k4_child_counts(it, R, nf, nn);
+        const SymCase c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
+        to_fin = symq_row(it.meta) + 1u >= symq_target(it.meta, R);
+        for (uint32_t f = it.f; f; f &= f - 1u) {
+            const uint32_t b = f & (0u - f);
+            if (symq_child(it, c, b).f) live |= b;
+        }
+        const uint32_t k = (uint32_t)__popc(live);
+        nf = to_fin ? k : 0u;
+        nn = to_fin ? 0u : k;
     }
     uint32_t pa, pb, ta, tb;
     k4_block_scan2(nf, nn, pa, pb, ta, tb, wsum);
-    if (nf | nn) {
-        const uint4 off = part[blockIdx.x];
-        uint32_t bf = fin_at + off.x + pa, bn = off.y + pb;
-        const SymCase c = sym_case(symq_n(it.meta), symq_b1(it.meta), symq_corner(it.meta));
-        const uint32_t tgt = symq_target(it.meta, R);
-        for (uint32_t f = it.f; f; f &= f - 1u) {
-            const SymQ ch = symq_child(it, c, f & (0u - f));
-            if (!ch.f) continue;
-            if (symq_row(ch.meta) >= tgt) {
-                if (bf < fin_cap)
-                    fin_q[bf] = ch;
-                else
-                    atomicOr(err, 1u);
-                ++bf;
-            } else {
-                if (bn < next_cap)
-                    next_q[bn] = ch;
-                else
-                    atomicOr(err, 1u);
-                ++bn;
-            }
+    // outputs of the block: its final children [0, ta), then its next ones
+    s_item[threadIdx.x] = it;
+    uint32_t o = to_fin ? pa : ta + pb;
+    for (uint32_t f = live; f; f &= f - 1u, ++o) {
+        s_map[o] = (uint8_t)threadIdx.x;
+        s_col[o] = (uint8_t)__builtin_ctz(f);
+    }
+    __syncthreads();
+    const uint4 off = part[blockIdx.x];
+    for (uint32_t k = threadIdx.x; k < ta + tb; k += 256u) {
+        const SymQ& p = s_item[s_map[k]];
+        const SymCase c = sym_case(symq_n(p.meta), symq_b1(p.meta), symq_corner(p.meta));
+        const SymQ ch = symq_child(p, c, 1u << s_col[k]);
+        if (k < ta) {
+            const uint32_t d = fin_at + off.x + k;
+            if (d < fin_cap)
+                fin_q[d] = ch;
+            else
+                atomicOr(err, 1u);
+        } else {
+            const uint32_t d = off.y + (k - ta);
+            if (d < next_cap)
+                next_q[d] = ch;
+            else
+                atomicOr(err, 1u);
         }
     }
 }
