@@ -216,7 +216,7 @@ struct FsDrainArgs {
     uint32_t n;
     uint32_t stack;         // branch points a lane can hold
     uint32_t refill_idle;   // a wave takes states when this many lanes are idle
-    uint32_t* head;         // states handed out
+    uint32_t* head;         // states handed out: kHeadSegs segments of the queue, a head each kHeadStride apart
     uint64_t* wave_part;    // per workgroup {solutions, placements}
     uint32_t* err;          // bit 2: a stack overflowed
 };
@@ -383,17 +383,33 @@ __global__ void __launch_bounds__(kFsDrainThreads) fs_drain_kernel(FsDrainArgs a
     const uint32_t n = a.n, M = (uint32_t)((1ull << n) - 1ull);
     FsLane L{};
     bool drained = false;
+    // the queue in kHeadSegs segments with a head each (no head that every
+    // wave of the chip updates); a wave starts at its own and moves on when it
+    // runs out
+    auto seg_lo = [&](uint32_t sg) { return (uint32_t)((uint64_t)a.count * sg / kHeadSegs); };
+    uint32_t seg = blockIdx.x % kHeadSegs, seg_done = 0;
     for (;;) {
         const uint64_t idle = __ballot(!L.act);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!drained && (nidle >= a.refill_idle || nidle == 64u)) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(a.head, nidle);
-            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)base, 0, 64));
-            if (base >= a.count) drained = true;
+            uint32_t base = 0, got = 0;
+            while (got == 0 && !drained) {  // wave-uniform, at most kHeadSegs rounds
+                const uint32_t lo = seg_lo(seg), len = seg_lo(seg + 1u) - lo;
+                uint32_t g = 0;
+                if (lane == 0) g = atomicAdd(a.head + seg * kHeadStride, nidle);
+                g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)g, 0, 64));
+                if (g < len) {
+                    base = lo + g;
+                    got = min(nidle, len - g);
+                }
+                if (got < nidle) {  // this segment is used up
+                    if (++seg_done == kHeadSegs) drained = true;
+                    seg = seg + 1u == kHeadSegs ? 0u : seg + 1u;
+                }
+            }
             if (!L.act) {
-                const uint32_t idx = base + fs_rank(idle);
-                if (idx < a.count) fs_lane_start(L, a.items[idx], M);
+                const uint32_t r = fs_rank(idle);
+                if (r < got) fs_lane_start(L, a.items[base + r], M);
             }
         }
         if (drained && __ballot(L.act) == 0) break;
@@ -448,8 +464,9 @@ extern "C" int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_
     uint64_t host_nodes = roots.size();
 
     int rc;
-    // counters: acc (kFsAccSlots x 256 B), cursor at byte 16384, err at 16388, head at 16640
-    const size_t ctr_bytes = 16384 + 512;
+    // counters: acc (kFsAccSlots x 256 B), cursor at byte 16384, err at 16388, the queue's
+    // heads from byte 16640 (kHeadSegs x kHeadStride u32)
+    const size_t ctr_bytes = 16640 + (size_t)kHeadSegs * kHeadStride * 4u;
     if ((rc = ensure_dev(ctx, W.ctr, ctr_bytes)) != NQM_OK) return rc;
     unsigned long long* d_acc = static_cast<unsigned long long*>(W.ctr.ptr);
     uint32_t* d_cursor = reinterpret_cast<uint32_t*>(static_cast<char*>(W.ctr.ptr) + 16384);
@@ -505,7 +522,7 @@ extern "C" int nqm_count_fullsweep_ex(nqm_ctx* ctx, uint32_t n, uint32_t target_
         a.count = count;
         a.n = n;
         a.stack = stack;
-        a.refill_idle = 16u;
+        a.refill_idle = 4u;  // few lanes idle at a time (the queue's heads are segmented)
         a.head = d_head;
         a.wave_part = static_cast<uint64_t*>(W.wave.ptr);
         a.err = d_err;
