@@ -404,7 +404,11 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     ".Lk4_nd" #U "_%=:\n"                                  \
     "s_mov_b64 exec, %[alive]\n"                           \
     "s_waitcnt lgkmcnt(0)\n"
-#define NQM_K4_STEP(U, CX, CA, CL, CR, CT, NX, NA, NL, NR) \
+// AM(NA): the corner rule's "column 1 back at row b1" (NQM_K4_AM_ON), or
+// nothing in a phase none of whose lanes has that rule (NQM_K4_AM_OFF)
+#define NQM_K4_AM_ON(NA) "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"
+#define NQM_K4_AM_OFF(NA) ""
+#define NQM_K4_STEP(U, CX, CA, CL, CR, CT, NX, NA, NL, NR, AM) \
     "v_sub_u32 %[ng], 0, " CX "\n"                         \
     "v_and_b32 %[b], " CX ", %[ng]\n"                      \
     "v_or_b32 " NR ", " CR ", %[b]\n"                      \
@@ -420,7 +424,7 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_bfe_i32 %[t2v], " NR ", %[sn], 1\n"                 \
     "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
     "v_bitop3_b32 " NA ", " CA ", %[b], %[t2v] bitop3:0x14\n" \
-    "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"      \
+    AM(NA)                                                 \
     "v_bfe_i32 %[mt], " NA ", %[on], 1\n"                  \
     "v_bfe_i32 %[m0], " NA ", %[o0], 1\n"                  \
     "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
@@ -448,10 +452,10 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "s_add_u32 %[cnt], %[cnt], %[t]\n"                     \
     "s_mov_b64 exec, %[alive]\n"                           \
     ".Lk4_ne" #U "_%=:\n"
-#define NQM_K4_PASS_A(U) NQM_K4_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95")
-#define NQM_K4_PASS_B(U) NQM_K4_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91")
-#define NQM_K4_PASS8 NQM_K4_PASS_A(0) NQM_K4_PASS_B(1) NQM_K4_PASS_A(2) NQM_K4_PASS_B(3) \
-                     NQM_K4_PASS_A(4) NQM_K4_PASS_B(5) NQM_K4_PASS_A(6) NQM_K4_PASS_B(7)
+#define NQM_K4_PASS_A(U, AM) NQM_K4_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", AM)
+#define NQM_K4_PASS_B(U, AM) NQM_K4_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", AM)
+#define NQM_K4_PASS8(AM) NQM_K4_PASS_A(0, AM) NQM_K4_PASS_B(1, AM) NQM_K4_PASS_A(2, AM) NQM_K4_PASS_B(3, AM) \
+                         NQM_K4_PASS_A(4, AM) NQM_K4_PASS_B(5, AM) NQM_K4_PASS_A(6, AM) NQM_K4_PASS_B(7, AM)
 
 typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
 
@@ -469,6 +473,42 @@ struct K4Rules {
 // in hasnext whose piece ends starts its next item (nst, nr) in the same
 // pass, and its bit leaves hasnext. Returns the non-tie full boards (psols)
 // and the placements (pnodes) of the wave.
+#define NQM_K4_PHASE_ASM(AM) \
+    asm volatile( \
+        "s_mov_b32 %[sols], 0\n" \
+        "s_mov_b32 %[nodes], 0\n" \
+        "s_mov_b64 %[live], exec\n" \
+        "s_mov_b64 %[alive], exec\n" \
+        "s_mov_b32 %[ran], 0\n" \
+        ".Lk4_pass_%=:\n" NQM_K4_PASS8(AM) \
+        "s_add_u32 %[ran], %[ran], 8\n" \
+        "s_bcnt1_i32_b64 %[t], exec\n" \
+        "s_cmp_le_u32 %[t], %[floor]\n" \
+        "s_cbranch_scc1 .Lk4_end_%=\n" \
+        "s_bcnt1_i32_b64 %[t], %[hasnext]\n" \
+        "s_cmp_le_i32 %[t], %[nfloor]\n" \
+        "s_cbranch_scc1 .Lk4_end_%=\n" \
+        "s_cmp_ge_u32 %[cnt], %[clim]\n" \
+        "s_cbranch_scc1 .Lk4_end_%=\n" \
+        "s_cmp_lt_u32 %[ran], %[lim]\n" \
+        "s_cbranch_scc1 .Lk4_pass_%=\n" \
+        ".Lk4_end_%=:\n" \
+        "s_bcnt1_i32_b64 %[t], exec\n" \
+        "s_mul_i32 %[t], %[t], %[ran]\n" \
+        "s_add_u32 %[nodes], %[nodes], %[t]\n" \
+        "s_mov_b64 exec, %[live]\n" \
+        "s_waitcnt lgkmcnt(0)\n" \
+        : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:99]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng), \
+          [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live), \
+          [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran), \
+          [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c), [lm] "+v"(lm), [bitb1] "+v"(bitb1), \
+          [endb] "+v"(endb), [amask] "+v"(amask), [o0] "+v"(o0), [on] "+v"(on), [hasnext] "+s"(hasnext) \
+        : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [sn] "s"(sn), \
+          [sside] "s"(sside), [recbase] "s"(recbase), [nfloor] "s"(nfloor), \
+          "{v[100:103]}"(n4), "{v[104:111]}"(n8) \
+        : "vcc", "scc", "memory"); \
+
+template <bool kAmask>
 __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, const uint4& nst, const K4Rules& nr,
                                          uint64_t& hasnext, int32_t nfloor, uint32_t& cnt,
                                          uint64_t recbase, uint32_t sn, uint32_t sside, uint32_t floor_live,
@@ -484,39 +524,11 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, co
     uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
     uint32_t sols = 0, nodes = 0;
     uint32_t c = cnt;
-    asm volatile(
-        "s_mov_b32 %[sols], 0\n"
-        "s_mov_b32 %[nodes], 0\n"
-        "s_mov_b64 %[live], exec\n"
-        "s_mov_b64 %[alive], exec\n"
-        "s_mov_b32 %[ran], 0\n"
-        ".Lk4_pass_%=:\n" NQM_K4_PASS8
-        "s_add_u32 %[ran], %[ran], 8\n"
-        "s_bcnt1_i32_b64 %[t], exec\n"
-        "s_cmp_le_u32 %[t], %[floor]\n"
-        "s_cbranch_scc1 .Lk4_end_%=\n"
-        "s_bcnt1_i32_b64 %[t], %[hasnext]\n"
-        "s_cmp_le_i32 %[t], %[nfloor]\n"
-        "s_cbranch_scc1 .Lk4_end_%=\n"
-        "s_cmp_ge_u32 %[cnt], %[clim]\n"
-        "s_cbranch_scc1 .Lk4_end_%=\n"
-        "s_cmp_lt_u32 %[ran], %[lim]\n"
-        "s_cbranch_scc1 .Lk4_pass_%=\n"
-        ".Lk4_end_%=:\n"
-        "s_bcnt1_i32_b64 %[t], exec\n"
-        "s_mul_i32 %[t], %[t], %[ran]\n"
-        "s_add_u32 %[nodes], %[nodes], %[t]\n"
-        "s_mov_b64 exec, %[live]\n"
-        "s_waitcnt lgkmcnt(0)\n"
-        : "+{v[88:91]}"(s4), "={v[92:95]}"(s5), "+{v[96:99]}"(rec), [q] "+v"(q), [b] "=&v"(b), [ng] "=&v"(ng),
-          [e] "=&v"(e), [t1] "=&v"(t1), [t2v] "=&v"(t2v), [m0] "=&v"(m0), [mt] "=&v"(mt), [live] "=&s"(live),
-          [alive] "=&s"(alive), [sm] "=&s"(sm), [se] "=&s"(se), [t] "=&s"(t), [t2] "=&s"(t2), [ran] "=&s"(ran),
-          [sols] "=&s"(sols), [nodes] "=&s"(nodes), [cnt] "+s"(c), [lm] "+v"(lm), [bitb1] "+v"(bitb1),
-          [endb] "+v"(endb), [amask] "+v"(amask), [o0] "+v"(o0), [on] "+v"(on), [hasnext] "+s"(hasnext)
-        : [row] "i"(kK4Row), [floor] "s"(floor_live), [lim] "s"(lim32), [clim] "i"(kRecLimit), [sn] "s"(sn),
-          [sside] "s"(sside), [recbase] "s"(recbase), [nfloor] "s"(nfloor),
-          "{v[100:103]}"(n4), "{v[104:111]}"(n8)
-        : "vcc", "scc", "memory");
+    if constexpr (kAmask) {
+        NQM_K4_PHASE_ASM(NQM_K4_AM_ON)
+    } else {
+        NQM_K4_PHASE_ASM(NQM_K4_AM_OFF)
+    }
     (void)s5;
     rl = K4Rules{lm, bitb1, endb, amask, o0, on, rec.x};
     st = make_uint4(s4.x, s4.y, s4.z, s4.w);
@@ -709,7 +721,10 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
                 }
             }
         }
-        hasnext = __ballot(holdnext);
+        // the phase leaves out the corner rule's instruction when no lane's
+        // piece has that rule; a next item with it then starts after the phase
+        const bool amask_wave = __ballot(holds() && rl.amask != 0u) != 0;
+        hasnext = __ballot(holdnext && (amask_wave || nrl.amask == 0u));
 
         const uint64_t live_mask = __ballot(holds());
         const bool have_items = queue_open || lq < lq_end;
@@ -738,8 +753,12 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
         if (holds()) {
             const uint64_t had = hasnext;
-            k4_phase(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
-                     psols, pnodes, ran);
+            if (amask_wave)
+                k4_phase<true>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
+                               psols, pnodes, ran);
+            else
+                k4_phase<false>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
+                                psols, pnodes, ran);
             // the phase cleared the bits of the lanes that started their next item
             if (((had & ~hasnext) >> lane) & 1u) holdnext = false;
             if ((int32_t)q < 0) {  // the piece ended: its last load is void
