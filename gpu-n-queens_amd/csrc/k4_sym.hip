@@ -404,10 +404,14 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     ".Lk4_nd" #U "_%=:\n"                                  \
     "s_mov_b64 exec, %[alive]\n"                           \
     "s_waitcnt lgkmcnt(0)\n"
-// AM(NA): the corner rule's "column 1 back at row b1" (NQM_K4_AM_ON), or
-// nothing in a phase none of whose lanes has that rule (NQM_K4_AM_OFF)
-#define NQM_K4_AM_ON(NA) "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"
-#define NQM_K4_AM_OFF(NA) ""
+// AM(NA, NL): the corner rule's "column 1 back at row b1" (NQM_K4_AM_ON), or
+// nothing in a phase none of whose lanes has that rule (NQM_K4_AM_OFF);
+// NQM_K4_CAM_ON is NQM_K4_AM_ON for NQM_K4_CSTEP, which has no t1 of its own
+#define NQM_K4_AM_ON(NA, NL) "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"
+#define NQM_K4_AM_OFF(NA, NL) ""
+#define NQM_K4_CAM_ON(NA, NL)                              \
+    "v_ashrrev_i32 %[t1], 31, " NL "\n"                    \
+    "v_and_or_b32 " NA ", %[amask], %[t1], " NA "\n"
 #define NQM_K4_STEP(U, CX, CA, CL, CR, CT, NX, NA, NL, NR, AM) \
     "v_sub_u32 %[ng], 0, " CX "\n"                         \
     "v_and_b32 %[b], " CX ", %[ng]\n"                      \
@@ -424,7 +428,7 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "v_bfe_i32 %[t2v], " NR ", %[sn], 1\n"                 \
     "v_and_b32 %[t2v], %[lm], %[t2v]\n"                    \
     "v_bitop3_b32 " NA ", " CA ", %[b], %[t2v] bitop3:0x14\n" \
-    AM(NA)                                                 \
+    AM(NA, NL)                                             \
     "v_bfe_i32 %[mt], " NA ", %[on], 1\n"                  \
     "v_bfe_i32 %[m0], " NA ", %[o0], 1\n"                  \
     "v_bitop3_b32 %[mt], %[mt], %[m0], %[sside] bitop3:0xd4\n" \
@@ -452,10 +456,31 @@ __device__ __forceinline__ uint32_t k4_wave_max(uint32_t v) {
     "s_add_u32 %[cnt], %[cnt], %[t]\n"                     \
     "s_mov_b64 exec, %[alive]\n"                           \
     ".Lk4_ne" #U "_%=:\n"
-#define NQM_K4_PASS_A(U, AM) NQM_K4_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") NQM_K4_STEP(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", AM)
-#define NQM_K4_PASS_B(U, AM) NQM_K4_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") NQM_K4_STEP(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", AM)
-#define NQM_K4_PASS8(AM) NQM_K4_PASS_A(0, AM) NQM_K4_PASS_B(1, AM) NQM_K4_PASS_A(2, AM) NQM_K4_PASS_B(3, AM) \
-                         NQM_K4_PASS_A(4, AM) NQM_K4_PASS_B(5, AM) NQM_K4_PASS_A(6, AM) NQM_K4_PASS_B(7, AM)
+// A step of a phase none of whose lanes holds a piece with the row-b2, last-row
+// or tie rules (a corner item: lm = bitb1 = endb = 0, o0 = on = 31): avail' =
+// (avail ^ b) | (amask & t1), the free set avail' & ~ld' & ~rd', and every full
+// board a non-tie one.
+#define NQM_K4_CSTEP(U, CX, CA, CL, CR, CT, NX, NA, NL, NR, AM) \
+    "v_sub_u32 %[ng], 0, " CX "\n"                         \
+    "v_and_b32 %[b], " CX ", %[ng]\n"                      \
+    "v_or_b32 " NR ", " CR ", %[b]\n"                      \
+    "v_add_lshl_u32 " NL ", %[b], " CL ", 1\n"             \
+    "v_cmp_eq_u32 %[sm], " CA ", %[b]\n"                   \
+    "v_cmpx_ne_u32 vcc, " CX ", %[b]\n"                    \
+    "v_xor_b32 " CX ", " CX ", %[b]\n"                     \
+    "ds_write_b128 %[q], " CT " offset:%[row]\n"           \
+    "v_add_u32 %[q], %[row], %[q]\n"                       \
+    "s_mov_b64 exec, %[alive]\n"                           \
+    "v_lshrrev_b32 " NR ", 1, " NR "\n"                    \
+    "v_xor_b32 " NA ", " CA ", %[b]\n"                     \
+    AM(NA, NL)                                             \
+    "v_bitop3_b32 " NX ", " NA ", " NL ", " NR " bitop3:0x10\n" \
+    "s_bcnt1_i32_b64 %[t], %[sm]\n"                        \
+    "s_add_u32 %[sols], %[sols], %[t]\n"
+#define NQM_K4_PASS_A(U, ST, AM) NQM_K4_POP(U, "v88", "v89", "v90", "v91", "v[88:91]") ST(U, "v88", "v89", "v90", "v91", "v[88:91]", "v92", "v93", "v94", "v95", AM)
+#define NQM_K4_PASS_B(U, ST, AM) NQM_K4_POP(U, "v92", "v93", "v94", "v95", "v[92:95]") ST(U, "v92", "v93", "v94", "v95", "v[92:95]", "v88", "v89", "v90", "v91", AM)
+#define NQM_K4_PASS8(ST, AM) NQM_K4_PASS_A(0, ST, AM) NQM_K4_PASS_B(1, ST, AM) NQM_K4_PASS_A(2, ST, AM) NQM_K4_PASS_B(3, ST, AM) \
+                             NQM_K4_PASS_A(4, ST, AM) NQM_K4_PASS_B(5, ST, AM) NQM_K4_PASS_A(6, ST, AM) NQM_K4_PASS_B(7, ST, AM)
 
 typedef uint32_t k4_v4 __attribute__((ext_vector_type(4)));
 
@@ -473,14 +498,14 @@ struct K4Rules {
 // in hasnext whose piece ends starts its next item (nst, nr) in the same
 // pass, and its bit leaves hasnext. Returns the non-tie full boards (psols)
 // and the placements (pnodes) of the wave.
-#define NQM_K4_PHASE_ASM(AM) \
+#define NQM_K4_PHASE_ASM(ST, AM) \
     asm volatile( \
         "s_mov_b32 %[sols], 0\n" \
         "s_mov_b32 %[nodes], 0\n" \
         "s_mov_b64 %[live], exec\n" \
         "s_mov_b64 %[alive], exec\n" \
         "s_mov_b32 %[ran], 0\n" \
-        ".Lk4_pass_%=:\n" NQM_K4_PASS8(AM) \
+        ".Lk4_pass_%=:\n" NQM_K4_PASS8(ST, AM) \
         "s_add_u32 %[ran], %[ran], 8\n" \
         "s_bcnt1_i32_b64 %[t], exec\n" \
         "s_cmp_le_u32 %[t], %[floor]\n" \
@@ -508,7 +533,7 @@ struct K4Rules {
           "{v[100:103]}"(n4), "{v[104:111]}"(n8) \
         : "vcc", "scc", "memory"); \
 
-template <bool kAmask>
+template <bool kAmask, bool kCorner>
 __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, const uint4& nst, const K4Rules& nr,
                                          uint64_t& hasnext, int32_t nfloor, uint32_t& cnt,
                                          uint64_t recbase, uint32_t sn, uint32_t sside, uint32_t floor_live,
@@ -524,10 +549,14 @@ __device__ __forceinline__ void k4_phase(uint4& st, uint32_t& q, K4Rules& rl, co
     uint32_t b, ng, e, t1, t2v, m0, mt, t, t2;
     uint32_t sols = 0, nodes = 0;
     uint32_t c = cnt;
-    if constexpr (kAmask) {
-        NQM_K4_PHASE_ASM(NQM_K4_AM_ON)
+    if constexpr (kCorner && kAmask) {
+        NQM_K4_PHASE_ASM(NQM_K4_CSTEP, NQM_K4_CAM_ON)
+    } else if constexpr (kCorner) {
+        NQM_K4_PHASE_ASM(NQM_K4_CSTEP, NQM_K4_AM_OFF)
+    } else if constexpr (kAmask) {
+        NQM_K4_PHASE_ASM(NQM_K4_STEP, NQM_K4_AM_ON)
     } else {
-        NQM_K4_PHASE_ASM(NQM_K4_AM_OFF)
+        NQM_K4_PHASE_ASM(NQM_K4_STEP, NQM_K4_AM_OFF)
     }
     (void)s5;
     rl = K4Rules{lm, bitb1, endb, amask, o0, on, rec.x};
@@ -722,9 +751,12 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
             }
         }
         // the phase leaves out the corner rule's instruction when no lane's
-        // piece has that rule; a next item with it then starts after the phase
+        // piece has that rule, and the other rules when every piece is a
+        // corner item's; a next item that needs what the phase left out
+        // starts after the phase
         const bool amask_wave = __ballot(holds() && rl.amask != 0u) != 0;
-        hasnext = __ballot(holdnext && (amask_wave || nrl.amask == 0u));
+        const bool corner_wave = __ballot(holds() && rl.o0 == 0u) == 0;
+        hasnext = __ballot(holdnext && (amask_wave || nrl.amask == 0u) && (!corner_wave || nrl.o0 != 0u));
 
         const uint64_t live_mask = __ballot(holds());
         const bool have_items = queue_open || lq < lq_end;
@@ -753,12 +785,18 @@ __global__ void __launch_bounds__(kK4Threads) k4_drain_kernel(K4Args a) {
         uint32_t psols = 0, pnodes = 0, ran = 0, c = __builtin_amdgcn_readfirstlane(cnt);
         if (holds()) {
             const uint64_t had = hasnext;
-            if (amask_wave)
-                k4_phase<true>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
-                               psols, pnodes, ran);
+            if (corner_wave && amask_wave)
+                k4_phase<true, true>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live,
+                                     limit, psols, pnodes, ran);
+            else if (corner_wave)
+                k4_phase<false, true>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live,
+                                      limit, psols, pnodes, ran);
+            else if (amask_wave)
+                k4_phase<true, false>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live,
+                                      limit, psols, pnodes, ran);
             else
-                k4_phase<false>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live, limit,
-                                psols, pnodes, ran);
+                k4_phase<false, false>(st, q, rl, nst, nrl, hasnext, nfloor, c, recbase, a.n, a.side, floor_live,
+                                       limit, psols, pnodes, ran);
             // the phase cleared the bits of the lanes that started their next item
             if (((had & ~hasnext) >> lane) & 1u) holdnext = false;
             if ((int32_t)q < 0) {  // the piece ended: its last load is void
