@@ -24,7 +24,7 @@
 //    in the stack slots and in donated rows like the real bits. The
 //    item's ld is cut to n bits, so real diagonal bits stay below the markers
 //    for n <= 21; the stack of last - R rows fits LDS at 16 waves per CU for
-//    n <= 19 (k4_supports).
+//    n <= 19 and at 12 for n = 20 (k4_supports).
 //    A full board whose class may have another representative (a tie of the
 //    top queen's corner distance: the last queen on column b2, the column-0
 //    queen on row b2 -- which puts bit b1 into the last row's ld -- or row
@@ -233,7 +233,7 @@ k4_expand_scan_kernel(uint4* __restrict__ part, uint32_t nblocks, uint32_t* __re
 // All children of an item go to one queue (they share a row). The block
 // stages its items and, per output, the item and column in LDS; then thread k
 // writes outputs k, k + 256, ...: consecutive threads, consecutive items.
-constexpr uint32_t kK4MaxChildren = 20;  // columns of a row (n <= 19) and one to spare
+constexpr uint32_t kK4MaxChildren = 20;  // columns of a row below row 0 (n - 1 <= 19)
 __global__ void __launch_bounds__(256)
 k4_expand_write_kernel(const SymQ* __restrict__ in, uint32_t count, uint32_t R, const uint4* __restrict__ part,
                        SymQ* __restrict__ fin_q, uint32_t fin_at, uint32_t fin_cap, SymQ* __restrict__ next_q,
@@ -298,7 +298,7 @@ constexpr uint32_t kRecChunk = 4096;  // records per chunk of a wave
 // passes to the next vote add at most 8 x 64.
 constexpr uint32_t kRecLimit = kRecChunk - 8u * 64u;
 constexpr uint32_t kK4Grab = 128;  // queue items one wave takes per atomic at most
-constexpr uint32_t kK4MaxRows = 9;  // rows below an item (last - R, k4_supports)
+constexpr uint32_t kK4MaxRows = 10;  // rows below an item (last - R, k4_supports)
 
 struct K4Args {
     const SymQ* items;
@@ -1264,13 +1264,15 @@ void k4_host_rec(K4Host& h, const SymCase& c, uint32_t y, uint32_t l, uint32_t d
 uint32_t k4_rows_done(uint32_t n) { return (n & 1u) ? (n - 1u) / 2u : n / 2u - 1u; }
 
 bool k4_supports(uint32_t n) {
-    if (n < 10 || n > 21) return false;
+    // n = 21 would pass the checks below, but its tie records (about 7.5 times
+    // n = 20's 1.3e9) outgrow the 32-bit record indices
+    if (n < 10 || n > 20) return false;
     const uint32_t R = k4_rows_done(n), last = n - 1u;
     // the rd marker n + (last - R) fits in 32 bits; real ld bits (at most
     // n - 1 + (last - R)) stay below bit 31, and below the row-b2 marker,
     // which starts 31 - (b2 - R) bits up with b2 <= n - 2; the stack of
-    // last - R rows fits LDS at 16 waves per CU, and a tie search's rows
-    // R+1 .. last-1 its kTieRows entries
+    // last - R rows fits LDS (16 waves per CU up to 9 rows, 12 at 10), and a
+    // tie search's rows R+1 .. last-1 its kTieRows entries
     return n + (last - R) <= 31u && n - 1u + (last - R) < 31u && n - 2u < 32u - n + R && last - R <= kK4MaxRows &&
            last - R - 1u <= kTieRows;
 }
@@ -1292,7 +1294,7 @@ static int grow_keep(nqm_ctx* ctx, DevBuf& b, size_t used, size_t bytes) {
 
 int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats) {
     const double t0 = now_ms();
-    if (!k4_supports(n)) return fail(ctx, NQM_EUNSUPPORTED, "the symmetric drain takes 10 <= n <= 19, not %u", n);
+    if (!k4_supports(n)) return fail(ctx, NQM_EUNSUPPORTED, "the symmetric drain takes 10 <= n <= 20, not %u", n);
     NQM_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     SymWork& w = ctx->sym;
@@ -1443,7 +1445,8 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
 
     // record chunks: as many as the last count of this n used, else an
     // estimate from the queue (a run that runs out sets err bit 64 and is
-    // repeated with 4x more)
+    // repeated with as many as its chunk grabs asked for, at least twice as
+    // many)
     uint64_t nchunks = w.chunks ? w.chunks + w.chunks / 4u + 64u : (uint64_t)nwaves + nfin64 / 4096u + 64u;
     uint64_t leaves = 0, dnodes = 0, tie_u = 0, tie_t = 0, xnodes = 0;
     float search_ms = 0.f, all_ms = 0.f;
@@ -1490,9 +1493,9 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             NQM_HIP(ctx, hipStreamSynchronize(st));
             if (err & 1u) return fail(ctx, NQM_EDEVICE, "symmetric expansion overflowed its counted size");
             if (err & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
-            if (err & 64u) {
+            if (err & 64u) {  // the chunk head counts every grab, those past the end too
                 if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
-                nchunks *= 4u;
+                nchunks = std::max<uint64_t>(nchunks * 2u, (uint64_t)hc[0] + hc[0] / 8u + 64u);
                 NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
                 continue;
             }
@@ -1555,7 +1558,7 @@ int k4_count(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_st
             if (err2 & 16u) return fail(ctx, NQM_EDEVICE, "symmetric drain: dynamic LDS not at address 0");
             if (err2 & 64u) {  // out of record chunks: the tie stage ran on partial records; again with more
                 if (attempt >= 3) return fail(ctx, NQM_ERANGE, "tie records exceed %llu chunks", (unsigned long long)nchunks);
-                nchunks *= 4u;
+                nchunks = std::max<uint64_t>(nchunks * 2u, (uint64_t)hc2[0] + hc2[0] / 8u + 64u);
                 w.nrec_known = false;
                 NQM_HIP(ctx, hipMemsetAsync(d_err, 0, 4, st));
                 continue;

@@ -45,8 +45,8 @@
 namespace nqm {
 
 // Stack entries a search holds at most: rows r0+1 .. last-1 (r0 is the
-// drain's item row, ceil(n/2) - 1; 8 for n <= 19).
-constexpr uint32_t kTieRows = 8;
+// drain's item row, ceil(n/2) - 1; 9 for n <= 20).
+constexpr uint32_t kTieRows = 9;
 
 struct TieSearch {
     // the record and its item

@@ -389,7 +389,7 @@ Context.solve_batch = _solve_batch
 
 SYM_AUTO = 0
 SYM_THREAD = 1  # one thread per host item (any n)
-SYM_DRAIN = 2   # the persistent drain (10 <= n <= 19)
+SYM_DRAIN = 2   # the persistent drain (10 <= n <= 20)
 
 
 def _count_symmetric(ctx, n, path=None):

@@ -295,7 +295,7 @@ int nqm_autotune(nqm_ctx* ctx, const nqm_solve* solves, uint32_t nsolves,
 int nqm_count_symmetric(nqm_ctx* ctx, uint32_t n, uint64_t* unique, uint64_t* total, nqm_stats* stats);
 
 /* The same count by a chosen kernel: NQM_SYM_THREAD = one thread per host
- * item (any n), NQM_SYM_DRAIN = the persistent drain (10 <= n <= 19, else
+ * item (any n), NQM_SYM_DRAIN = the persistent drain (10 <= n <= 20, else
  * NQM_EUNSUPPORTED), NQM_SYM_AUTO = the drain where it applies (what
  * nqm_count_symmetric does). Every path gives the same unique, total and
  * stats->nodes. stats->search_ms = the drain (or thread) kernel alone,
